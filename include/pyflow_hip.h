@@ -1,0 +1,186 @@
+/*
+ * pyflow_hip.h -- C ABI of libpyflow_hip.so, the gfx950 HIP implementation of
+ * PyFlow's projection time step.
+ *
+ * Conventions
+ *   - Every function returns an int status: PF_OK (0) or a negative PF_E* code;
+ *     pf_last_error() returns a thread-local message for the last failure.
+ *   - Non-convergence and NaN/Inf are reported in output structs, never as
+ *     errors (reference: linear_solvers/interface.py:51-53, solvers/solver.py:116-121).
+ *   - Device buffers are owned by the caller: pf_batch_create() carves every
+ *     vector out of one caller-provided device workspace and never frees it.
+ *   - Host field arrays use the reference layout: float64, row-major
+ *     (ny+2) x (nx+2) including the ghost ring (core/ghost_fields.py:47-60),
+ *     or (ny) x (nx) interiors when ghost == 0; `ld` is the row stride in
+ *     elements. Flat cell index k = j*nx + i (solvers/pressure_solver.py:19).
+ *   - One batch = a set of independent cavities of arbitrary sizes advanced
+ *     together; a single cavity is a batch of one.
+ */
+#ifndef PYFLOW_HIP_H
+#define PYFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_ABI_VERSION 1
+
+#define PF_OK 0
+#define PF_EINVAL (-1)
+#define PF_ENOMEM (-2)
+#define PF_EHIP (-3)
+#define PF_ESTATE (-4)
+
+/* Arithmetic of the N-vectors. FP64: every vector and stencil in float64
+ * (parity form). MIXED: fields, stencils, SpMV and axpy in float32; dot
+ * products, norms, the right-hand side and an outer iterative-refinement
+ * residual/solution in float64. */
+enum { PF_PREC_FP64 = 0, PF_PREC_MIXED = 1 };
+
+/* Advection scheme (reference: solvers/solver.py:75-86). */
+enum { PF_SCHEME_UPWIND = 0, PF_SCHEME_QUICK = 1, PF_SCHEME_NONE = 2 };
+
+/* Pressure operator (reference: solvers/pressure_solver.py:61-106). */
+enum { PF_OP_NEG_LAP_PINNED = 0, PF_OP_POS_LAP_DIRICHLET = 1 };
+
+/* Standalone float64 operators (pf_op_apply). */
+enum {
+  PF_OPK_GRAD_COHERENT = 0,   /* in0=p          -> out0=gx, out1=gy  (numerics/core_ops_coherent.py:45-61) */
+  PF_OPK_DIV_COHERENT = 1,    /* in0=u,in1=v    -> out0              (core_ops_coherent.py:63-79) */
+  PF_OPK_LAP_COHERENT = 2,    /* in0=p          -> out0              (core_ops_coherent.py:81-84) */
+  PF_OPK_DIV_CENTRAL = 3,     /* in0=u,in1=v    -> out0              (numerics/fluid_ops.py:8-28) */
+  PF_OPK_ADV_QUICK = 4,       /* in0=u,in1=v,in2=f -> out0           (numerics/operators/advection.py:45-84) */
+  PF_OPK_ADV_UPWIND = 5,      /* in0=u,in1=v,in2=f -> out0           (advection.py:17-43) */
+  PF_OPK_NEG_LAP_PINNED = 6,  /* in0=x          -> out0              (pressure_solver.py:97-105) */
+  PF_OPK_POS_LAP_DIRICHLET = 7/* in0=x          -> out0              (pressure_solver.py:69-77) */
+};
+
+/* What a batch run executes for each cavity. */
+enum {
+  PF_MODE_STEP = 0,     /* full step(): dt control, predictor, projection, lid BC, residuals */
+  PF_MODE_PROJECT = 1,  /* solve_pressure_poisson() normal mode with a given dt */
+  PF_MODE_SOLVE = 2     /* linear_solvers.solve(A, b, 'cg') on a PyFlow operator, x0 = 0 */
+};
+
+/* Per-cavity parameters; mirrors the attributes step() reads through getattr
+ * (reference: solvers/solver.py:16-183, solvers/pressure_solver.py:58-146). */
+typedef struct pf_cavity_params {
+  int32_t nx, ny;          /* interior cells; both >= 2 */
+  int32_t scheme;          /* PF_SCHEME_* */
+  int32_t lid_enabled;     /* lid_velocity is not None */
+  int32_t jacobi;          /* enable_jacobi_pc */
+  int32_t lin_maxiter;
+  int32_t op;              /* PF_OP_* */
+  int32_t mode;            /* PF_MODE_* */
+  double lid_velocity;
+  double dx, dy;           /* lx/(nx-1), ly/(ny-1) */
+  double nu;               /* 1/Re */
+  double cfl_target, cfl_growth;
+  double lin_tol;
+  double inv_diag;         /* 1/(2/dx**2 + 2/dy**2): interior Jacobi weight */
+  double inner_rtol;       /* MIXED only: relative tolerance of each FP32 inner PCG pass */
+} pf_cavity_params;
+
+/* Per-cavity results of the last completed step / projection / solve. */
+typedef struct pf_step_diag {
+  double dt, cfl, umax, vmax;      /* dt controller outputs (solver.py:25-62) */
+  double dt_prev;                  /* == dt after a step; persisted for restart */
+  double hist[5];                  /* divergence_history (last <=5 values) */
+  int32_t hist_len;
+  int32_t safeguard;               /* dt halved by the divergence-rise safeguard */
+  double div_before;               /* ||central_div(u,v;1,1)|| at step start */
+  double div_initial;              /* ||div_c(u*)|| before projection */
+  double div_norm;                 /* ||div_c(u)|| after correction */
+  double rhs_mean;                 /* mean subtracted from the Poisson RHS */
+  double bnorm;                    /* ||b|| of the solved system */
+  double rp_residual;              /* true ||b - A x|| */
+  double ru, rv, continuity;       /* solver.py:108-114 */
+  int32_t rp_info;                 /* SciPy info: 0 or lin_maxiter */
+  int32_t rp_iterations;           /* PCG iterations actually executed (sum over passes) */
+  int32_t rp_converged;            /* info == 0 and rnorm <= tol*||b|| */
+  int32_t nan_detected;            /* any non-finite u, v, p */
+  int32_t outer_passes;            /* MIXED: iterative-refinement passes */
+  int32_t steps_done;              /* steps completed by the last pf_batch_run */
+} pf_step_diag;
+
+typedef struct pf_batch pf_batch;
+
+int pf_abi_version(void);
+const char* pf_last_error(void);
+/* sizeof(pf_cavity_params) and sizeof(pf_step_diag), for binding layout checks. */
+size_t pf_sizeof_params(void);
+size_t pf_sizeof_diag(void);
+
+/* Select the HIP device for the calling thread (hipSetDevice). */
+int pf_device_init(int device);
+
+/* Bytes of device workspace a batch needs. */
+int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* params, size_t* bytes);
+
+/* Create a batch over `ncav` cavities. `workspace` is a device allocation of at
+ * least pf_batch_workspace_bytes() bytes (256-byte aligned); `stream` is a
+ * hipStream_t (NULL = default stream). Fields start at zero, dt_prev at 1e-2. */
+int pf_batch_create(int precision, int ncav, const pf_cavity_params* params,
+                    void* workspace, size_t workspace_bytes, void* stream, pf_batch** out);
+int pf_batch_destroy(pf_batch* b);
+
+/* Dynamic batches (continuous batching): an empty batch with room for
+ * `max_cav` cavities and `max_tiles` 64x4-cell tiles in total; cavities are
+ * added with their own workspace (pf_batch_workspace_bytes for ncav = 1) and
+ * removed between runs. Slots are the indices used by every per-cavity call. */
+int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* stream, pf_batch** out);
+int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, size_t bytes, int* slot);
+int pf_batch_remove(pf_batch* b, int slot);
+
+/* Replace the parameters of cavity `cav` (same nx, ny). */
+int pf_batch_set_params(pf_batch* b, int cav, const pf_cavity_params* params);
+
+/* Host float64 <-> device fields of one cavity. Any of u, v, p may be NULL. */
+int pf_batch_upload(pf_batch* b, int cav, const double* u, const double* v, const double* p,
+                    int64_t ld, int ghost);
+int pf_batch_download(pf_batch* b, int cav, double* u, double* v, double* p,
+                      int64_t ld, int ghost);
+/* Zero u, v, p of every cavity (allocate_state()). */
+int pf_batch_zero_fields(pf_batch* b);
+
+/* dt controller memory: state.meta['dt_prev'] and ['divergence_history']. */
+int pf_batch_set_meta(pf_batch* b, int cav, double dt_prev, const double* hist, int hist_len);
+
+/* Right-hand side for PF_MODE_SOLVE (float64, ny x nx, row stride ld). */
+int pf_batch_set_rhs(pf_batch* b, int cav, const double* rhs, int64_t ld);
+/* Solution of the last solve/projection in float64 (x before any pin offset). */
+int pf_batch_get_solution(pf_batch* b, int cav, double* x, int64_t ld);
+
+/* dt used by PF_MODE_PROJECT. */
+int pf_batch_set_dt(pf_batch* b, int cav, double dt);
+
+/* Run: every cavity whose `active` flag is non-zero (NULL = all) executes
+ * `nsteps` steps (PF_MODE_STEP) or one projection/solve; blocks until all are
+ * done, then writes one pf_step_diag per cavity into `diag` (may be NULL). */
+int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* diag);
+
+/* Start the same work without waiting (kernels queued on the batch stream
+ * in chunks); pf_batch_poll() returns 1 when finished, 0 if not yet. */
+int pf_batch_launch(pf_batch* b, int nsteps, const int32_t* active);
+int pf_batch_poll(pf_batch* b, int* done);
+int pf_batch_diag(pf_batch* b, pf_step_diag* diag);
+
+/* u[:, nx//2] of one cavity as float64 (drivers/simulation_driver.py:90-96). */
+int pf_batch_centerline(pf_batch* b, int cav, double* out);
+
+/* Rounds (kernel launches) executed by the last run; for profiling. */
+int pf_batch_rounds(pf_batch* b, int64_t* rounds);
+
+/* Standalone float64 operator on device arrays of shape (ny, nx), dense rows.
+ * Unused inputs/outputs may be NULL. */
+int pf_op_apply(int opk, int nx, int ny, double dx, double dy,
+                const double* in0, const double* in1, const double* in2,
+                double* out0, double* out1, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYFLOW_HIP_H */

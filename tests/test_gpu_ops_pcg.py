@@ -1,0 +1,130 @@
+"""GPU parity: standalone float64 operator kernels are bit-identical to the
+oracle (and hence to the reference); the device PCG follows SciPy's cg."""
+import numpy as np
+import pytest
+
+from oracle import pyflow_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _ops():
+    from pyflow._ops import apply_op
+    return apply_op
+
+
+SHAPES = [(2, 2), (3, 3), (2, 7), (5, 2), (5, 9), (8, 8), (16, 12), (7, 4), (37, 130), (129, 65)]
+
+
+@pytest.mark.parametrize("ny,nx", SHAPES)
+def test_operator_kernels_bitwise(ny, nx):
+    apply_op = _ops()
+    rng = np.random.default_rng(ny * 1000 + nx)
+    u, v, f = (rng.standard_normal((ny, nx)) for _ in range(3))
+    u[rng.random((ny, nx)) < 0.2] = 0.0
+    v[rng.random((ny, nx)) < 0.2] = 0.0
+    dx, dy = 0.37, 1.3
+    print(f"[ops] shape=({ny},{nx})", flush=True)
+    assert np.array_equal(apply_op("adv_quick", (ny, nx), dx, dy, u, v, f), O.advect_quick(u, v, f, dx, dy))
+    assert np.array_equal(apply_op("adv_upwind", (ny, nx), dx, dy, u, v, f), O.advect_upwind(u, v, f, dx, dy))
+    assert np.array_equal(apply_op("div_central", (ny, nx), dx, dy, u, v), O.divergence_central(u, v, dx, dy))
+    gx, gy = apply_op("grad_coherent", (ny, nx), dx, dy, f, nout=2)
+    ogx, ogy = O.gradient_coherent(f, dx, dy)
+    assert np.array_equal(gx, ogx) and np.array_equal(gy, ogy)
+    assert np.array_equal(apply_op("div_coherent", (ny, nx), dx, dy, u, v), O.divergence_coherent(u, v, dx, dy))
+    assert np.array_equal(apply_op("lap_coherent", (ny, nx), dx, dy, f), O.laplacian_coherent(f, dx, dy))
+    x = f.reshape(-1)
+    assert np.array_equal(apply_op("neg_lap_pinned", (ny, nx), dx, dy, x).reshape(-1),
+                          O.neg_laplacian_pinned(x, nx, ny, dx, dy))
+    assert np.array_equal(apply_op("pos_lap_dirichlet", (ny, nx), dx, dy, x).reshape(-1),
+                          O.pos_laplacian_dirichlet(x, nx, ny, dx, dy))
+
+
+def test_numerics_api_on_golden_vectors(golden_ops):
+    from pyflow.numerics.core_ops_coherent import divergence_coherent, gradient_coherent, laplacian_coherent
+    from pyflow.numerics.fluid_ops import divergence
+    from pyflow.numerics.operators.advection import advect_quick, advect_upwind
+    g = golden_ops
+    for k in range(int(g["n_ops"])):
+        u, v, f = g[f"ops{k}_u"], g[f"ops{k}_v"], g[f"ops{k}_f"]
+        dx, dy = g[f"ops{k}_dxdy"]
+        assert np.array_equal(advect_quick(u, v, f, dx, dy), g[f"ops{k}_quick"])
+        assert np.array_equal(advect_upwind(u, v, f, dx, dy), g[f"ops{k}_upwind"])
+        assert np.array_equal(divergence(u, v, dx, dy), g[f"ops{k}_cdiv"])
+        gx, gy = gradient_coherent(f, dx, dy)
+        assert np.array_equal(gx, g[f"ops{k}_gx"]) and np.array_equal(gy, g[f"ops{k}_gy"])
+        assert np.array_equal(divergence_coherent(u, v, dx, dy), g[f"ops{k}_hdiv"])
+        assert np.array_equal(laplacian_coherent(f, dx, dy), g[f"ops{k}_lap"])
+
+
+def _rhs(nx, ny, seed):
+    b = np.random.default_rng(seed).standard_normal(nx * ny)
+    b -= b.mean()
+    b[0] = 0.0
+    return b
+
+
+@pytest.mark.parametrize("nx,ny,pc,maxiter", [(24, 20, True, 300), (24, 20, False, 300), (65, 33, True, 40),
+                                              (130, 97, True, 2000), (3, 3, True, 10)])
+def test_device_cg_fp64_follows_scipy_recurrence(nx, ny, pc, maxiter):
+    from pyflow.linear_solvers.interface import PinnedNegLaplacian, device_cg
+    b = _rhs(nx, ny, nx + ny)
+    op = PinnedNegLaplacian(nx, ny, 1.0, 1.0)
+    print(f"[cg] {nx}x{ny} pc={pc} maxiter={maxiter}", flush=True)
+    x, info, its, rnorm, conv = device_cg(op, b, 1e-10, maxiter, jacobi=pc, precision="fp64")
+    mv = lambda z: O.neg_laplacian_pinned(z, nx, ny, 1.0, 1.0)  # noqa: E731
+    invD = O.jacobi_inv_diag(nx, ny, 1.0, 1.0)
+    xo, info_o, its_o = O.cg(mv, b, 1e-10, maxiter, (lambda r: invD * r) if pc else None)
+    assert info == info_o
+    assert abs(its - its_o) <= 1
+    scale = np.max(np.abs(xo))
+    assert np.max(np.abs(x - xo)) <= 1e-9 * scale
+    r_o = float(np.linalg.norm(b - mv(xo)))
+    assert abs(rnorm - r_o) <= 1e-6 * max(r_o, 1e-300) + 1e-14 * np.linalg.norm(b)
+
+
+def test_solve_api_and_edge_cases():
+    from pyflow.linear_solvers.interface import DirichletPosLaplacian, PinnedNegLaplacian, solve
+    from pyflow.linear_solvers.preconditioners import jacobi_preconditioner
+    from pyflow.solvers.pressure_solver import assemble_negative_laplacian
+    nx, ny = 20, 16
+    op = PinnedNegLaplacian(nx, ny, 1.0, 1.0)
+    b = _rhs(nx, ny, 4)
+    M = jacobi_preconditioner(assemble_negative_laplacian(nx, ny, 1.0, 1.0))
+    r = solve(op, b, method="cg", tol=1e-10, maxiter=500, M=M)
+    assert r.converged and r.iterations == 0 and r.residual_norm <= 1e-10 * np.linalg.norm(b)
+    # zero RHS: x = b, info 0
+    r0 = solve(op, np.zeros(nx * ny), method="cg", tol=1e-10, maxiter=50)
+    assert r0.converged and not r0.x.any() and r0.residual_norm == 0.0
+    # iteration cap: iterations reports maxiter
+    rc = solve(op, b, method="cg", tol=1e-14, maxiter=3, M=M)
+    assert not rc.converged and rc.iterations == 3
+    # NaN RHS runs to maxiter (SciPy: atol = max(0, rtol*nan) = 0)
+    bn = b.copy()
+    bn[5] = np.nan
+    rn = solve(op, bn, method="cg", tol=1e-10, maxiter=7)
+    assert rn.iterations == 7 and not rn.converged and np.isnan(rn.x).all()
+    # Dirichlet operator (manufactured mode operator)
+    opd = DirichletPosLaplacian(8, 8, 1 / 7, 1 / 7)
+    X, Y = np.meshgrid(np.linspace(0, 1, 8), np.linspace(0, 1, 8))
+    rhs = np.zeros((8, 8))
+    rhs[1:-1, 1:-1] = -2 * np.pi ** 2 * (np.sin(np.pi * X) * np.sin(np.pi * Y))[1:-1, 1:-1]
+    rd = solve(opd, rhs.reshape(-1), method="cg", tol=1e-10, maxiter=500)
+    xo, info_o, _ = O.cg(lambda z: O.pos_laplacian_dirichlet(z, 8, 8, 1 / 7, 1 / 7), rhs.reshape(-1), 1e-10, 500)
+    assert rd.converged
+    np.testing.assert_allclose(rd.x, xo, rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("n", [64, 200])
+def test_device_cg_mixed_reaches_fp64_tolerance(n):
+    """FP32 inner PCG + FP64 iterative refinement meets the FP64 stopping rule on
+    the true residual, and its solution matches the FP64 oracle."""
+    from pyflow.linear_solvers.interface import PinnedNegLaplacian, device_cg
+    b = _rhs(n, n, 7)
+    op = PinnedNegLaplacian(n, n, 1.0, 1.0)
+    for tol in (1e-6, 1e-10):
+        x, info, its, rnorm, conv = device_cg(op, b, tol, 20000, jacobi=True, precision="mixed")
+        assert info == 0 and conv
+        assert rnorm <= tol * np.linalg.norm(b)
+        mv = lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0)  # noqa: E731
+        assert float(np.linalg.norm(b - mv(x))) <= tol * np.linalg.norm(b) * 1.0001

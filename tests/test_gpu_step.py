@@ -1,0 +1,319 @@
+"""GPU parity of the full projection step against the reference's own outputs
+(golden trajectories written by running the reference) and the reference's
+behavioural tests restated against the GPU drop-in."""
+import io
+import json
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from oracle import pyflow_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# FP64 parity bound (relative to the field's max magnitude). The stencils are
+# bit-identical; only reduction order (dot products, norms, mean) differs from
+# NumPy, and CG amplifies those last-bit differences over its iterations.
+FP64_FIELD_RTOL = 1e-8
+FP64_SCALAR_RTOL = 1e-8
+
+STEP_KEYS_FLOAT = ["dt", "CFL", "Ru", "Rv", "continuity", "Rp_residual", "divergence_norm", "max_residual",
+                   "residual_drop_Ru_orders", "nu"]
+
+
+def _run_case(golden_steps, name, precision="fp64"):
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.residuals.manager import ResidualManager
+    from pyflow.solvers.solver import step
+    s = golden_steps
+    cfg = SimpleNamespace(precision=precision, **json.loads(str(s[f"st_{name}_cfg"])))
+    u0 = s[f"st_{name}_u0"]
+    ny, nx = u0.shape[0] - 2, u0.shape[1] - 2
+    st = allocate_state(nx, ny)
+    st.fields["u"][:] = u0
+    st.fields["v"][:] = s[f"st_{name}_v0"]
+    tr = ResidualManager()
+    rows = []
+    for it in range(s[f"st_{name}_diag"].shape[0]):
+        st, res, diag = step(cfg, st, tr, it)
+        rows.append({**res, **diag})
+    return st, rows
+
+
+def _rel(a, b):
+    scale = max(float(np.nanmax(np.abs(b))), 1e-300)
+    return float(np.nanmax(np.abs(a - b))) / scale
+
+
+@pytest.mark.parametrize("name", ["quick8", "upwind16", "quick12x10", "safeguard8", "noadv12x10", "nolid9x7",
+                                  "c1_64"])
+def test_step_trajectory_matches_reference_fp64(golden_steps, name):
+    s = golden_steps
+    keys = json.loads(str(s["step_keys"]))
+    print(f"[step] case={name}", flush=True)
+    st, rows = _run_case(golden_steps, name)
+    D = s[f"st_{name}_diag"]
+    for it, row in enumerate(rows):
+        exp = dict(zip(keys, D[it]))
+        assert row["Rp_iterations"] == int(exp["Rp_iterations"]), (it, row["Rp_iterations"], exp["Rp_iterations"])
+        assert row["Rp_converged"] == bool(exp["Rp_converged"])
+        assert row["nan_detected"] == bool(exp["nan_detected"])
+        for k in STEP_KEYS_FLOAT:
+            e = exp[k]
+            assert abs(row[k] - e) <= FP64_SCALAR_RTOL * abs(e) + 1e-300 or (e == 0 and abs(row[k]) < 1e-14), \
+                (name, it, k, row[k], e)
+    for f in ("u", "v", "p"):
+        ref = s[f"st_{name}_{f}"][-1]
+        got = st.fields[f]
+        assert got.shape == ref.shape
+        assert np.array_equal(got[0, :], ref[0, :]) and np.array_equal(got[:, 0], ref[:, 0])  # ghost ring
+        err = _rel(got, ref)
+        print(f"[step] {name} field {f} rel err {err:.3e}", flush=True)
+        assert err <= FP64_FIELD_RTOL, (name, f, err)
+
+
+def test_nan_propagation_matches_reference_fixture(golden_steps):
+    """NaN in u spreads exactly like the reference's emergency checkpoint:
+    all interior u except the lid row, all of v and p."""
+    st, rows = _run_case(golden_steps, "nan10")
+    assert rows[0]["nan_detected"] is True
+    assert rows[0]["Rp_iterations"] == 30
+    for f, exp in (("u", 90), ("v", 100), ("p", 100)):
+        assert int(np.isnan(st.fields[f]).sum()) == exp
+    assert np.array_equal(np.isnan(st.fields["u"]), np.isnan(golden_steps["st_nan10_u"][-1]))
+    assert np.all(st.fields["u"][-2, 1:-1] == 0.0)
+
+
+def test_nan_detection_logging_and_emergency_checkpoint(tmp_path, monkeypatch):
+    """tests/test_nan_detection_logging.py:10-22 restated."""
+    from pyflow.core.ghost_fields import allocate_state, interior_view
+    from pyflow.drivers.simulation_driver import SimulationDriver
+    from pyflow.residuals.manager import ResidualManager
+    monkeypatch.chdir(tmp_path)
+    buf = io.StringIO()
+    cfg = SimpleNamespace(disable_advection=True, advection_scheme="upwind", cfl_target=0.5, cfl_growth=1.05,
+                          Re=100.0, lid_velocity=0.0, lin_tol=1e-10, lin_maxiter=30, nx=10, ny=10, lx=9, ly=9,
+                          diagnostics=False, log_stream=buf)
+    state = allocate_state(cfg.nx, cfg.ny)
+    interior_view(state.fields["u"])[2, 2] = np.nan
+    for _st, _res, diag in SimulationDriver(cfg, state, ResidualManager()).run(max_steps=1):
+        assert diag.get("nan_detected") is True
+    lines = [l for l in buf.getvalue().splitlines() if l.strip()]
+    assert any('"reason":"nan_detected"' in l for l in lines)
+    assert (tmp_path / "emergency_FAIL_NAN_it0.npz").exists()
+
+
+def test_driver_yields_same_state_and_centerline():
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.drivers.simulation_driver import SimulationDriver
+    from pyflow.residuals.manager import ResidualManager
+
+    class Cfg:
+        disable_advection = False
+        advection_scheme = "quick"
+        cfl_target = 0.5
+        cfl_growth = 1.05
+        Re = 100.0
+        lid_velocity = 1.0
+        lin_tol = 1e-10
+        lin_maxiter = 50
+        nx = 8
+        ny = 8
+        lx = nx - 1
+        ly = ny - 1
+        diagnostics = False
+
+    cfg = Cfg()
+    state = allocate_state(cfg.nx, cfg.ny)
+    last_dt = None
+    n = 0
+    for st, residuals, diag in SimulationDriver(cfg, state, ResidualManager()).run(max_steps=5):
+        assert st is state
+        if last_dt is not None:
+            assert diag["dt"] <= last_dt * 1.5
+        last_dt = diag["dt"]
+        yc, uc = diag["u_centerline"]
+        assert len(yc) == len(uc) == 8
+        assert uc == [float(x) for x in state.fields["u"][1:-1, 1:-1][:, 4]]
+        n += 1
+    assert n == 5
+
+
+def test_lid_bc_and_divergence_reduced():
+    """tests/test_projection_method_contract.py:186-268 restated."""
+    from pyflow.core.ghost_fields import allocate_state, interior_view
+    from pyflow.numerics.fluid_ops import divergence
+    from pyflow.residuals.manager import ResidualManager
+    from pyflow.solvers.solver import step
+
+    class Cfg:
+        disable_advection = False
+        advection_scheme = "quick"
+        cfl_target = 0.5
+        cfl_growth = 1.1
+        Re = 100.0
+        lid_velocity = 1.0
+        lin_tol = 1e-10
+        lin_maxiter = 200
+        diagnostics = False
+
+    state = allocate_state(8, 8)
+    u = np.zeros((10, 10))
+    u[3:6, 3:6] = 1.0
+    state.fields["u"][:] = u
+    before = np.linalg.norm(divergence(interior_view(state.fields["u"]), interior_view(state.fields["v"]), 1.0, 1.0))
+    state, residuals, diagnostics = step(Cfg(), state, ResidualManager(), 0)
+    after = np.linalg.norm(divergence(interior_view(state.fields["u"]), interior_view(state.fields["v"]), 1.0, 1.0))
+    assert after < before
+    assert np.allclose(state.fields["u"][-2, 1:-1], 1.0)
+
+
+def test_safeguard_and_seed_reproducibility_bitwise():
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.residuals.manager import ResidualManager
+    from pyflow.solvers.solver import step
+
+    def run():
+        cfg = SimpleNamespace(nx=16, ny=16, Re=100.0, lid_velocity=1.0, cfl_target=0.5, cfl_growth=1.05,
+                              advection_scheme="upwind", disable_advection=False, lin_tol=1e-10, lin_maxiter=50,
+                              seed=42, diagnostics=False)
+        st = allocate_state(16, 16)
+        tr = ResidualManager()
+        for it in range(5):
+            st, _r, d = step(cfg, st, tr, it)
+        return st.fields["u"].copy(), st.fields["v"].copy(), st.fields["p"].copy(), d
+
+    a, b = run(), run()
+    for k in range(3):
+        assert np.array_equal(a[k], b[k])
+    assert a[3]["seed"] == 42
+
+
+def test_restart_is_bitwise(tmp_path):
+    """Checkpoint after N steps, restart, run M more: identical to N+M straight
+    (the dt controller's memory is persisted in the checkpoint meta)."""
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.drivers.simulation_driver import SimulationDriver
+    from pyflow.io.checkpoint import load_checkpoint, save_checkpoint
+    from pyflow.residuals.manager import ResidualManager
+    for precision in ("fp64", "mixed"):
+        cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05,
+                              Re=100.0, lid_velocity=1.0, lin_tol=1e-9, lin_maxiter=200, nx=20, ny=18,
+                              diagnostics=False, precision=precision)
+        full = allocate_state(20, 18)
+        for _ in SimulationDriver(cfg, full, ResidualManager()).run(max_steps=9):
+            pass
+        part = allocate_state(20, 18)
+        for _ in SimulationDriver(cfg, part, ResidualManager()).run(max_steps=5):
+            pass
+        ck = str(tmp_path / f"ck_{precision}.npz")
+        save_checkpoint(ck, part, 4, 0.0, cfg)
+        loaded, meta = load_checkpoint(ck)
+        for _ in SimulationDriver(cfg, loaded, ResidualManager()).run(max_steps=9, start_iteration=5):
+            pass
+        for f in ("u", "v", "p"):
+            assert np.array_equal(loaded.fields[f], full.fields[f]), (precision, f)
+        assert loaded.meta["dt_prev"] == full.meta["dt_prev"]
+
+
+def test_preconditioner_effect_and_manufactured_projection():
+    """tests/test_preconditioner_effect.py:9-33 and test_projection_coherent.py:11-45 restated."""
+    from pyflow.core.ghost_fields import allocate_state, interior_view
+    from pyflow.numerics.core_ops_coherent import divergence_coherent
+    from pyflow.solvers.pressure_solver import solve_pressure_poisson
+    nx, ny = 24, 20
+    out = {}
+    for pc in (True, False):
+        st = allocate_state(nx, ny)
+        interior_view(st.fields["u"])[ny // 3, nx // 3] = 1.0
+        cfg = SimpleNamespace(nx=nx, ny=ny, lin_tol=1e-10, lin_maxiter=300, diagnostics=False, enable_jacobi_pc=pc)
+        out[pc] = solve_pressure_poisson(st, 0.01, 1.0, 1.0, cfg)
+    assert np.max(np.abs(out[True][0] - out[False][0])) / np.max(np.abs(out[True][0])) < 1e-9
+    assert out[True][1]["Rp_true_iterations"] < out[False][1]["Rp_true_iterations"]
+    assert np.linalg.norm(out[True][0]) > 0
+    for nx, ny in ((8, 8), (12, 10)):
+        dx, dy = 1.0 / (nx - 1), 1.0 / (ny - 1)
+        X, Y = np.meshgrid(np.linspace(0, 1, nx), np.linspace(0, 1, ny))
+        dt = 0.01
+        st = allocate_state(nx, ny)
+        interior_view(st.fields["u"])[:] = dt * (-np.pi * np.sin(np.pi * X) * np.cos(np.pi * Y))
+        interior_view(st.fields["v"])[:] = dt * (-np.pi * np.cos(np.pi * X) * np.sin(np.pi * Y))
+        n0 = np.linalg.norm(divergence_coherent(interior_view(st.fields["u"]), interior_view(st.fields["v"]), dx, dy))
+        cfg = SimpleNamespace(lin_tol=1e-12, lin_maxiter=500, diagnostics=False)
+        p, diag = solve_pressure_poisson(st, dt, dx, dy, cfg)
+        n1 = np.linalg.norm(divergence_coherent(interior_view(st.fields["u"]), interior_view(st.fields["v"]), dx, dy))
+        assert n1 < max(1e-10, n0 * 5e-6)
+        pe = np.cos(np.pi * X) * np.cos(np.pi * Y)
+        pe -= pe.flat[0]
+        assert np.linalg.norm(p - pe) / np.linalg.norm(pe) < 0.25
+
+
+def test_pressure_solver_matches_reference_golden(golden_ops):
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.solvers.pressure_solver import solve_pressure_poisson
+    g = golden_ops
+    st = allocate_state(13, 11)
+    st.fields["u"][:] = g["ps_rand_u0"]
+    st.fields["v"][:] = g["ps_rand_v0"]
+    cfg = SimpleNamespace(lin_tol=1e-12, lin_maxiter=400, diagnostics=False)
+    p, d = solve_pressure_poisson(st, 0.05, 0.1, 0.2, cfg)
+    assert _rel(p, g["ps_rand_p"]) < 1e-9
+    assert _rel(st.fields["u"], g["ps_rand_u"]) < 1e-9
+    exp = g["ps_rand_diag"]
+    assert d["Rp_iterations"] == int(exp[0]) and d["Rp_converged"] == bool(exp[2])
+    assert abs(d["divergence_norm_initial"] - exp[5]) <= 1e-12 * exp[5]
+    st = allocate_state(8, 8)
+    cfg = SimpleNamespace(lin_tol=1e-10, lin_maxiter=500, diagnostics=False)
+    p, d = solve_pressure_poisson(st, 1.0, 1 / 7, 1 / 7, cfg, rhs_override=g["ps_mms_rhs"])
+    assert _rel(p, g["ps_mms_p"]) < 1e-9
+    assert d["divergence_norm_initial"] is None and d["divergence_reduction_factor"] is None
+    X, Y = np.meshgrid(np.linspace(0, 1, 8), np.linspace(0, 1, 8))
+    assert np.allclose(p[1:-1, 1:-1], (np.sin(np.pi * X) * np.sin(np.pi * Y))[1:-1, 1:-1], atol=5e-2)
+
+
+def test_mixed_precision_step_against_fp64_oracle():
+    """Mixed precision (FP32 fields/stencils/SpMV, FP64 reductions + refinement)
+    against the FP64 oracle: 12 QUICK steps on 96x96, tight tolerance."""
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.residuals.manager import ResidualManager
+    from pyflow.solvers.solver import step
+    n = 96
+    cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05,
+                          Re=1000.0, lid_velocity=1.0, lin_tol=1e-8, lin_maxiter=5000, diagnostics=False,
+                          precision="mixed")
+    st = allocate_state(n, n)
+    ref = O.allocate_state(n, n)
+    tr = ResidualManager()
+    for it in range(12):
+        st, res, diag = step(cfg, st, tr, it)
+        rres, rdiag = O.step(cfg, ref, it)
+        assert diag["Rp_converged"] == rdiag["Rp_converged"]
+        assert abs(diag["dt"] - rdiag["dt"]) <= 1e-6 * rdiag["dt"]
+    du = float(np.max(np.abs(st.fields["u"] - ref.fields["u"])))
+    dv = float(np.max(np.abs(st.fields["v"] - ref.fields["v"])))
+    dp = float(np.linalg.norm(st.fields["p"] - ref.fields["p"]) / np.linalg.norm(ref.fields["p"]))
+    print(f"[mixed] max|du|={du:.3e} max|dv|={dv:.3e} rel|dp|={dp:.3e} "
+          f"cont={res['continuity']:.6e} oracle={rres['continuity']:.6e}", flush=True)
+    assert du <= 1e-5 and dv <= 1e-5 and dp <= 1e-4
+    assert abs(res["continuity"] - rres["continuity"]) <= 1e-6
+
+
+def test_batch_alone_equals_in_mix():
+    """Per-cavity reductions depend only on that cavity: a cavity advanced
+    alone and inside a mixed batch of other sizes gives identical bits."""
+    from pyflow._device import DeviceBatch, cavity_params
+    mk = lambda n, re: cavity_params(n, n, dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=re,  # noqa: E731
+                                     lin_tol=1e-6, lin_maxiter=3000)
+    for prec in ("fp64", "mixed"):
+        alone = DeviceBatch([mk(48, 300.0)], prec)
+        da = alone.run(6)
+        ua = np.zeros((50, 50)); alone.download(0, ua, None, None)
+        mix = DeviceBatch([mk(32, 100.0), mk(48, 300.0), mk(64, 5000.0), mk(96, 1000.0)], prec)
+        dm = mix.run(6)
+        um = np.zeros((50, 50)); mix.download(1, um, None, None)
+        assert np.array_equal(ua, um), prec
+        assert da[0].rp_iterations == dm[1].rp_iterations and da[0].continuity == dm[1].continuity
+        assert all(d.steps_done == 6 for d in dm.values())
+        alone.close()
+        mix.close()
