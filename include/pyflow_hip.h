@@ -104,6 +104,8 @@ typedef struct pf_step_diag {
   int32_t nan_detected;            /* any non-finite u, v, p */
   int32_t outer_passes;            /* MIXED: iterative-refinement passes */
   int32_t steps_done;              /* steps completed by the last pf_batch_run */
+  int32_t rp_iterations_run;       /* PCG iterations summed over all steps of the last run */
+  int32_t outer_passes_run;        /* refinement passes summed over the last run */
 } pf_step_diag;
 
 typedef struct pf_batch pf_batch;
@@ -128,7 +130,7 @@ int pf_batch_create(int precision, int ncav, const pf_cavity_params* params,
 int pf_batch_destroy(pf_batch* b);
 
 /* Dynamic batches (continuous batching): an empty batch with room for
- * `max_cav` cavities and `max_tiles` 64x4-cell tiles in total; cavities are
+ * `max_cav` cavities and `max_tiles` 64x16-cell tiles in total; cavities are
  * added with their own workspace (pf_batch_workspace_bytes for ncav = 1) and
  * removed between runs. Slots are the indices used by every per-cavity call. */
 int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* stream, pf_batch** out);
@@ -173,6 +175,12 @@ int pf_batch_centerline(pf_batch* b, int cav, double* out);
 
 /* Rounds (kernel launches) executed by the last run; for profiling. */
 int pf_batch_rounds(pf_batch* b, int64_t* rounds);
+/* Optional timing: with timing enabled every chunk of back-to-back round
+ * launches is bracketed by hipEvents on the batch stream; pf_batch_timing
+ * returns the summed GPU milliseconds and the number of rounds they cover
+ * (host-side polling gaps excluded). Enabling resets both counters. */
+int pf_batch_set_timing(pf_batch* b, int enable);
+int pf_batch_timing(pf_batch* b, double* gpu_ms, int64_t* rounds);
 
 /* Standalone float64 operator on device arrays of shape (ny, nx), dense rows.
  * Unused inputs/outputs may be NULL. */

@@ -65,6 +65,30 @@ __device__ inline T apply_op5(int op, T c, T e, T w, T n, T s, int nx, int ny, i
   return lap;
 }
 
+// Float32 form of the same operators for the mixed-precision path: the
+// reflective-Neumann 5-point stencil ((e-c)-(c-w))*cx + ((n-c)-(c-s))*cy with
+// cx = 1/dx^2, cy = 1/dy^2 precomputed, missing neighbours' differences
+// dropped. Algebraically identical to lap_coherent5; rounds differently.
+__device__ inline float lap5_fast(float c, float e, float w, float n, float s, int nx, int ny, int j, int i,
+                                  float cx, float cy) {
+  if (nx < 2 || ny < 2) return 0.0f;
+  const float ex = (i < nx - 1) ? (e - c) : 0.0f;
+  const float wx = (i > 0) ? (c - w) : 0.0f;
+  const float ny_ = (j < ny - 1) ? (n - c) : 0.0f;
+  const float sy = (j > 0) ? (c - s) : 0.0f;
+  return __fmul_rn(ex - wx, cx) + __fmul_rn(ny_ - sy, cy);
+}
+__device__ inline float apply_op5_fast(int op, float c, float e, float w, float n, float s, int nx, int ny, int j,
+                                       int i, float cx, float cy) {
+  const float lap = lap5_fast(c, e, w, n, s, nx, ny, j, i, cx, cy);
+  if (op == 0) {
+    if (j == 0 && i == 0) return c;
+    return -lap;
+  }
+  if (j == 0 || j == ny - 1 || i == 0 || i == nx - 1) return c;
+  return lap;
+}
+
 // Gather the 5-point neighbourhood through an accessor and apply the operator.
 template <class T, class F>
 __device__ inline T apply_op_at(int op, F f, int nx, int ny, int j, int i, T dx, T dy) {

@@ -3,21 +3,21 @@
 // Execution model
 // ---------------
 // A batch holds cavities (independent lid-driven-cavity problems of any size).
-// Each cavity is cut into 64 x TY-cell tiles; one workgroup (64 x 4 threads,
-// one wave per row of 64 cells) owns one tile. Every cavity runs a phase
-// machine on the device:
+// Each cavity is cut into 64 x 16-cell tiles and covered by nblk workgroups of
+// 64 x 16 threads (one wave per tile row), each owning tpb consecutive tiles.
+// Every cavity runs a phase machine on the device:
 //
 //   STATS -> PRED_A -> PRED_B -> PINIT -> {P -> Q}* [-> OUTER_A -> OUTER_B]*
 //         -> FIN -> CORR_A -> CORR_B -> (STATS of the next step | DONE)
 //
-// A "round" is one launch over all tiles of all cavities; each tile executes
+// A "round" is one launch over all workgroups of all cavities; each executes
 // the phase its cavity is in. Phases only read data written in earlier rounds
-// (plus their own cells), so no tile waits on another inside a launch: there
-// is no spin-wait anywhere, and a round always drains. Per-cavity reductions
-// are deterministic: each tile writes fixed-order FP64 partials, the last tile
-// to arrive (atomic ticket) sums them in tile order and runs the cavity's
-// scalar tail (dt controller, CG alpha/beta, convergence tests), then advances
-// the cavity's phase for the next round.
+// (plus their own cells), so no workgroup waits on another inside a launch:
+// there is no spin-wait and no memory fence anywhere, and a round always
+// drains. Per-cavity reductions are deterministic: each workgroup writes
+// fixed-order FP64 partials; in the next round every workgroup of the cavity
+// sums them in the same fixed order and runs the cavity's scalar tail (dt
+// controller, CG alpha/beta, convergence tests) -- see pf_rounds.hpp.
 //
 // Reference correspondence (beautifulplanet/PyFlow-Solver, src/pyflow):
 //   STATS   solvers/solver.py:25-62      abs-max, central divergence, dt control
@@ -39,631 +39,8 @@
 #include <string>
 #include <vector>
 
-#include "pf_cells.hpp"
+#include "pf_rounds.hpp"
 #include "pyflow_hip.h"
-
-namespace pf {
-
-constexpr int TX = 64;        // tile width = wavefront width
-constexpr int TY = 4;         // tile height (rows); one row per wave
-constexpr int NTH = TX * TY;  // threads per workgroup
-constexpr int NWAVE = NTH / 64;
-constexpr int NP = 4;         // FP64 partials per tile
-
-enum Phase {
-  PH_IDLE = 0,
-  PH_STATS = 1,
-  PH_PRED_A = 2,
-  PH_PRED_B = 3,
-  PH_PINIT = 4,
-  PH_P = 5,
-  PH_Q = 6,
-  PH_OUTER_A = 7,
-  PH_OUTER_B = 8,
-  PH_FIN = 9,
-  PH_CORR_A = 10,
-  PH_CORR_B = 11,
-  PH_DONE = 12,
-  PH_COUNT = 13
-};
-
-// Read-mostly description of one cavity (rewritten only between runs).
-struct PfCav {
-  int nx, ny, tiles_x, ntiles, tile0;
-  int scheme, lid_on, jacobi, maxiter, op, mode, prec;
-  double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
-  void *u, *v, *p, *us, *vs, *r, *d0, *d1, *x;  // element type T
-  double *b, *x64;                              // x64 aliases x in FP64 mode
-  double *part;                                 // ntiles * NP partials
-};
-
-// Dynamic per-cavity scalars, owned by the device during a run.
-struct PfScal {
-  int phase, steps_left, steps_done, it, it_total, info, outer, x_is_b;
-  int hist_len, safeguard, nan, converged;
-  unsigned ticket;
-  int pad[3];
-  double dt_prev, dt, cfl_now, umax, vmax, hist[5], div_before;
-  double mean, div_initial, bnorm, atol, inner_atol, rho, rho_prev, beta, pq, alpha, rnorm2, rn_outer;
-  double rnorm_true, div_after, ru, rv, continuity;
-};
-
-template <class T>
-struct Grid {
-  const T* __restrict__ a;
-  int nx;
-  __device__ T operator()(int j, int i) const { return a[(size_t)j * nx + i]; }
-};
-
-__device__ inline int tid_flat() { return threadIdx.y * TX + threadIdx.x; }
-
-// Reduce K per-thread values over the workgroup (sum, or NaN-propagating max
-// where bit k of maxmask is set). Fixed butterfly + fixed wave order, so the
-// result is bitwise reproducible. Result valid in thread 0.
-template <int K>
-__device__ inline void block_reduce(double (&v)[K], unsigned maxmask, double (*sm)[NP]) {
-  const int tid = tid_flat();
-  const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double o = __shfl_xor(v[k], off, 64);
-      v[k] = ((maxmask >> k) & 1u) ? nanmax(v[k], o) : v[k] + o;
-    }
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) sm[wave][k] = v[k];
-  }
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double a = sm[0][k];
-      for (int w = 1; w < NWAVE; ++w) a = ((maxmask >> k) & 1u) ? nanmax(a, sm[w][k]) : a + sm[w][k];
-      v[k] = a;
-    }
-  }
-  __syncthreads();
-}
-
-// Write this tile's partials, take a ticket; the last tile of the cavity sums
-// all partials in tile order into tot[] (valid in thread 0) and returns true.
-template <int K>
-__device__ bool reduce_arrive(const PfCav& C, PfScal* S, int lt, double (&v)[K], unsigned maxmask,
-                              double (&tot)[K]) {
-  __shared__ double sm[NWAVE][NP];
-  __shared__ int is_last;
-  const int tid = tid_flat();
-  block_reduce<K>(v, maxmask, sm);
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) C.part[(size_t)lt * NP + k] = v[k];
-    __threadfence();
-    const unsigned tk = atomicAdd(&S->ticket, 1u);
-    is_last = (tk == (unsigned)(C.ntiles - 1));
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double a[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) a[k] = 0.0;
-  for (int t = tid; t < C.ntiles; t += NTH) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double q = C.part[(size_t)t * NP + k];
-      a[k] = ((maxmask >> k) & 1u) ? nanmax(a[k], q) : a[k] + q;
-    }
-  }
-  block_reduce<K>(a, maxmask, sm);
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) tot[k] = a[k];
-    S->ticket = 0u;
-  }
-  return true;
-}
-
-// Arrival without partials: returns true in every thread of the last tile.
-__device__ bool arrive_only(const PfCav& C, PfScal* S) {
-  __shared__ int is_last;
-  const int tid = tid_flat();
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    const unsigned tk = atomicAdd(&S->ticket, 1u);
-    is_last = (tk == (unsigned)(C.ntiles - 1));
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  if (tid == 0) S->ticket = 0u;
-  return true;
-}
-
-__device__ inline void enter_done(PfScal* S, int* active) {
-  S->phase = PH_DONE;
-  atomicSub(active, 1);
-}
-
-// ---------------------------------------------------------------------------
-// Phase bodies. Each is executed by every thread of a tile; (j, i) is this
-// thread's cell, `in` whether it lies inside the cavity.
-// ---------------------------------------------------------------------------
-template <class T>
-__device__ void ph_stats(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  Grid<T> U{(const T*)C.u, C.nx}, V{(const T*)C.v, C.nx};
-  double v[3] = {0.0, 0.0, 0.0};
-  if (in) {
-    v[0] = fabs((double)U(j, i));
-    v[1] = fabs((double)V(j, i));
-    const double cd = (double)div_central<T>(U, V, C.nx, C.ny, j, i, T(1), T(1));
-    v[2] = cd * cd;
-  }
-  double tot[3];
-  if (!reduce_arrive<3>(C, S, lt, v, 0x3u, tot)) return;
-  if (tid_flat() != 0) return;
-  // dt controller, solver.py:25-62 (Python float semantics)
-  const double umax = tot[0] + 1e-12;
-  const double vmax = tot[1] + 1e-12;
-  const double dt_cfl = pymin(C.cfl * C.dx / umax, C.cfl * C.dy / vmax);
-  const double dt_diff = 0.25 * pymin(C.dx * C.dx, C.dy * C.dy) / pymax(C.nu, 1e-12);
-  const double dt_cand = pymin(dt_cfl, dt_diff);
-  const double cap = S->dt_prev * C.growth;
-  double dt = (dt_cand > cap) ? cap : dt_cand;
-  const double div0 = sqrt(tot[2]);
-  int hl = S->hist_len;
-  if (hl == 5) {
-    for (int k = 0; k < 4; ++k) S->hist[k] = S->hist[k + 1];
-    S->hist[4] = div0;
-  } else {
-    S->hist[hl] = div0;
-    hl += 1;
-    S->hist_len = hl;
-  }
-  int sg = 0;
-  if (hl >= 3) {
-    const double a = S->hist[hl - 3], b = S->hist[hl - 2], c = S->hist[hl - 1];
-    if (a > 0 && b > a * 1.01 && c > b * 1.01) {
-      dt *= 0.5;
-      sg = 1;
-    }
-  }
-  S->safeguard = sg;
-  S->dt = dt;
-  S->dt_prev = dt;
-  S->umax = umax;
-  S->vmax = vmax;
-  S->div_before = div0;
-  S->cfl_now = pymax(umax * dt / C.dx, vmax * dt / C.dy);
-  S->phase = PH_PRED_A;
-}
-
-template <class T>
-__device__ void ph_pred_a(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  if (in) {
-    Grid<T> U{(const T*)C.u, C.nx}, V{(const T*)C.v, C.nx};
-    const size_t k = (size_t)j * C.nx + i;
-    const T uc = U(j, i), vc = V(j, i);
-    T us = uc, vs = vc;
-    if (C.scheme != PF_SCHEME_NONE) {
-      const T dt = (T)S->dt, dx = (T)C.dx, dy = (T)C.dy;
-      T cu, cv;
-      if (C.scheme == PF_SCHEME_QUICK) {
-        cu = adv_quick<T>(uc, vc, U, C.nx, C.ny, j, i, dx, dy);
-        cv = adv_quick<T>(uc, vc, V, C.nx, C.ny, j, i, dx, dy);
-      } else {
-        cu = adv_upwind<T>(uc, vc, U, C.nx, C.ny, j, i, dx, dy);
-        cv = adv_upwind<T>(uc, vc, V, C.nx, C.ny, j, i, dx, dy);
-      }
-      us = uc - dt * cu;
-      vs = vc - dt * cv;
-    }
-    ((T*)C.us)[k] = us;
-    ((T*)C.vs)[k] = vs;
-  }
-  (void)lt;
-  if (!arrive_only(C, S)) return;
-  if (tid_flat() == 0) S->phase = PH_PRED_B;
-}
-
-template <class T>
-__device__ void ph_pred_b(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  double v[2] = {0.0, 0.0};
-  if (in) {
-    Grid<T> US{(const T*)C.us, C.nx}, VS{(const T*)C.vs, C.nx};
-    const T dv = div_coherent<T>(US, VS, C.nx, C.ny, j, i, (T)C.dx, (T)C.dy);
-    const double rhs = (-(double)dv) / pymax(S->dt, 1e-14);
-    C.b[(size_t)j * C.nx + i] = rhs;
-    v[0] = rhs;
-    v[1] = (double)dv * (double)dv;
-  }
-  double tot[2];
-  if (!reduce_arrive<2>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  S->mean = tot[0] / (double)((size_t)C.nx * C.ny);
-  S->div_initial = sqrt(tot[1]);
-  S->phase = PH_PINIT;
-}
-
-// Decide what follows a completed inner iteration / initialisation.
-__device__ inline void cg_next(const PfCav& C, PfScal* S, int* active_unused) {
-  (void)active_unused;
-  const bool mixed = (C.prec == PF_PREC_MIXED);
-  if (S->it_total >= C.maxiter) {
-    S->info = C.maxiter;
-    S->phase = mixed ? PH_OUTER_A : PH_FIN;
-    return;
-  }
-  if (sqrt(S->rnorm2) < S->inner_atol) {
-    if (!mixed) S->info = 0;
-    S->phase = mixed ? PH_OUTER_A : PH_FIN;
-    return;
-  }
-  S->phase = PH_P;
-}
-
-template <class T>
-__device__ void ph_pinit(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  double v[3] = {0.0, 0.0, 0.0};
-  if (in) {
-    const size_t k = (size_t)j * C.nx + i;
-    double bb = C.b[k];
-    if (C.mode != PF_MODE_SOLVE) {
-      bb = bb - S->mean;
-      if (k == 0) bb = 0.0;
-      C.b[k] = bb;
-    }
-    const T rr = (T)bb;
-    ((T*)C.r)[k] = rr;
-    ((T*)C.x)[k] = T(0);
-    if (C.prec == PF_PREC_MIXED) C.x64[k] = 0.0;
-    const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rr;
-    v[0] = bb * bb;
-    v[1] = (double)rr * (double)z;
-    v[2] = (double)rr * (double)rr;
-  }
-  double tot[3];
-  if (!reduce_arrive<3>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  const double bn = sqrt(tot[0]);
-  S->bnorm = bn;
-  S->atol = pymax(0.0, C.tol * bn);
-  S->it = 0;
-  S->it_total = 0;
-  S->outer = 0;
-  S->info = 0;
-  S->x_is_b = 0;
-  if (bn == 0.0) {  // scipy returns x = b, info 0
-    S->x_is_b = 1;
-    S->phase = PH_FIN;
-    return;
-  }
-  S->rho = tot[1];
-  S->rnorm2 = tot[2];
-  S->rn_outer = bn;
-  if (C.maxiter <= 0) {  // loop never runs: info = maxiter (= 0 or negative)
-    S->info = C.maxiter;
-    S->phase = PH_FIN;
-    return;
-  }
-  if (C.prec == PF_PREC_MIXED) {
-    if (bn < S->atol) {
-      S->phase = PH_FIN;
-      return;
-    }
-    S->inner_atol = pymax(C.inner_rtol * bn, 0.5 * S->atol);
-    S->phase = PH_P;
-    return;
-  }
-  S->inner_atol = S->atol;
-  if (sqrt(S->rnorm2) < S->atol) {
-    S->phase = PH_FIN;
-    return;
-  }
-  S->phase = PH_P;
-}
-
-// Search direction: d = z (first inner iteration) or (d_old*beta) + z.
-template <class T>
-__device__ inline T dnew_at(const PfCav& C, const T* __restrict__ R, const T* __restrict__ Dold, int first,
-                            double beta, int j, int i) {
-  const size_t k = (size_t)j * C.nx + i;
-  const T rr = R[k];
-  const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rr;
-  if (first) return z;
-  return (T)((double)Dold[k] * beta + (double)z);
-}
-
-template <class T>
-__device__ void ph_p(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  const int it = S->it;
-  const T* R = (const T*)C.r;
-  const T* Dold = (const T*)((it & 1) ? C.d0 : C.d1);
-  T* Dnew = (T*)((it & 1) ? C.d1 : C.d0);
-  const int first = (it == 0);
-  const double beta = S->beta;
-  double v[1] = {0.0};
-  if (in) {
-    const int nx = C.nx, ny = C.ny;
-    const T c = dnew_at<T>(C, R, Dold, first, beta, j, i);
-    const T e = (i + 1 < nx) ? dnew_at<T>(C, R, Dold, first, beta, j, i + 1) : T(0);
-    const T w = (i > 0) ? dnew_at<T>(C, R, Dold, first, beta, j, i - 1) : T(0);
-    const T n = (j + 1 < ny) ? dnew_at<T>(C, R, Dold, first, beta, j + 1, i) : T(0);
-    const T s = (j > 0) ? dnew_at<T>(C, R, Dold, first, beta, j - 1, i) : T(0);
-    Dnew[(size_t)j * nx + i] = c;
-    const T q = apply_op5<T>(C.op, c, e, w, n, s, nx, ny, j, i, (T)C.dx, (T)C.dy);
-    v[0] = (double)c * (double)q;
-  }
-  double tot[1];
-  if (!reduce_arrive<1>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  S->pq = tot[0];
-  S->alpha = S->rho / tot[0];
-  S->phase = PH_Q;
-}
-
-template <class T>
-__device__ void ph_q(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  const int it = S->it;
-  const T* D = (const T*)((it & 1) ? C.d1 : C.d0);
-  const double alpha = S->alpha;
-  double v[2] = {0.0, 0.0};
-  if (in) {
-    Grid<T> DG{D, C.nx};
-    const size_t k = (size_t)j * C.nx + i;
-    const T q = apply_op_at<T>(C.op, DG, C.nx, C.ny, j, i, (T)C.dx, (T)C.dy);
-    T* X = (T*)C.x;
-    T* R = (T*)C.r;
-    const T d = D[k];
-    X[k] = (T)((double)X[k] + alpha * (double)d);
-    const T rn = (T)((double)R[k] - alpha * (double)q);
-    R[k] = rn;
-    const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rn;
-    v[0] = (double)rn * (double)z;
-    v[1] = (double)rn * (double)rn;
-  }
-  double tot[2];
-  if (!reduce_arrive<2>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  S->rho_prev = S->rho;
-  S->rho = tot[0];
-  S->rnorm2 = tot[1];
-  S->beta = S->rho / S->rho_prev;
-  S->it += 1;
-  S->it_total += 1;
-  cg_next(C, S, nullptr);
-}
-
-// MIXED: fold the inner correction into the FP64 solution.
-template <class T>
-__device__ void ph_outer_a(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  if (in) {
-    const size_t k = (size_t)j * C.nx + i;
-    T* X = (T*)C.x;
-    C.x64[k] = C.x64[k] + (double)X[k];
-    X[k] = T(0);
-  }
-  (void)lt;
-  if (!arrive_only(C, S)) return;
-  if (tid_flat() == 0) S->phase = PH_OUTER_B;
-}
-
-// MIXED: FP64 residual of the refined solution, restart the FP32 inner PCG.
-template <class T>
-__device__ void ph_outer_b(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  double v[3] = {0.0, 0.0, 0.0};
-  if (in) {
-    const size_t k = (size_t)j * C.nx + i;
-    Grid<double> XG{C.x64, C.nx};
-    const double ax = apply_op_at<double>(C.op, XG, C.nx, C.ny, j, i, C.dx, C.dy);
-    const double r64 = C.b[k] - ax;
-    const T rt = (T)r64;
-    ((T*)C.r)[k] = rt;
-    const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rt;
-    v[0] = r64 * r64;
-    v[1] = (double)rt * (double)z;
-    v[2] = (double)rt * (double)rt;
-  }
-  double tot[3];
-  if (!reduce_arrive<3>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  const double rn = sqrt(tot[0]);
-  S->rn_outer = rn;
-  S->outer += 1;
-  if (S->info != 0) {  // iteration budget exhausted
-    S->phase = PH_FIN;
-    return;
-  }
-  if (rn < S->atol) {
-    S->info = 0;
-    S->phase = PH_FIN;
-    return;
-  }
-  S->rho = tot[1];
-  S->rnorm2 = tot[2];
-  S->it = 0;
-  S->inner_atol = pymax(C.inner_rtol * rn, 0.5 * S->atol);
-  S->phase = PH_P;
-}
-
-template <class T>
-__device__ void ph_fin(const PfCav& C, PfScal* S, int lt, int j, int i, bool in, int* active) {
-  double v[1] = {0.0};
-  if (in) {
-    const size_t k = (size_t)j * C.nx + i;
-    const double* XS = S->x_is_b ? C.b : C.x64;
-    Grid<double> XG{XS, C.nx};
-    const double xv = XS[k];
-    const double ax = apply_op_at<double>(C.op, XG, C.nx, C.ny, j, i, C.dx, C.dy);
-    const double rr = C.b[k] - ax;
-    v[0] = rr * rr;
-    const double pv = (C.mode == PF_MODE_SOLVE) ? xv : (xv - XS[0]);
-    ((T*)C.p)[k] = (T)pv;
-  }
-  double tot[1];
-  if (!reduce_arrive<1>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  const double rn = sqrt(tot[0]);
-  S->rnorm_true = rn;
-  const double bn = (S->bnorm == 0.0) ? 1.0 : S->bnorm;
-  S->converged = (S->info == 0) && (rn <= C.tol * bn);
-  if (C.mode == PF_MODE_SOLVE) {
-    S->steps_done += 1;
-    enter_done(S, active);
-  } else {
-    S->phase = PH_CORR_A;
-  }
-}
-
-template <class T>
-__device__ void ph_corr_a(const PfCav& C, PfScal* S, int lt, int j, int i, bool in) {
-  double v[3] = {0.0, 0.0, 0.0};
-  if (in) {
-    const int nx = C.nx, ny = C.ny;
-    const size_t k = (size_t)j * nx + i;
-    Grid<T> PG{(const T*)C.p, nx};
-    const T dt = (T)S->dt;
-    const T gx = grad_x_coherent<T>(PG, nx, ny, j, i, (T)C.dx);
-    const T gy = grad_y_coherent<T>(PG, nx, ny, j, i, (T)C.dy);
-    T* US = (T*)C.us;
-    T* VS = (T*)C.vs;
-    const T uc = US[k] - dt * gx;
-    const T vc = VS[k] - dt * gy;
-    US[k] = uc;
-    VS[k] = vc;
-    const T uf = (C.lid_on && j == ny - 1) ? (T)C.lid : uc;
-    T* U = (T*)C.u;
-    T* V = (T*)C.v;
-    const double du = (double)uf - (double)U[k];
-    const double dv = (double)vc - (double)V[k];
-    U[k] = uf;
-    V[k] = vc;
-    v[0] = du * du;
-    v[1] = dv * dv;
-    const bool bad = !isfinite((double)uf) || !isfinite((double)vc) || !isfinite((double)PG(j, i));
-    v[2] = bad ? 1.0 : 0.0;
-  }
-  double tot[3];
-  if (!reduce_arrive<3>(C, S, lt, v, 0x4u, tot)) return;
-  if (tid_flat() != 0) return;
-  const double nn = (double)((size_t)C.nx * C.ny);
-  const double den = pymax(1.0, nn);
-  S->ru = sqrt(tot[0]) / den;
-  S->rv = sqrt(tot[1]) / den;
-  S->nan = (tot[2] != 0.0) ? 1 : 0;
-  S->phase = PH_CORR_B;
-}
-
-template <class T>
-__device__ void ph_corr_b(const PfCav& C, PfScal* S, int lt, int j, int i, bool in, int* active) {
-  double v[2] = {0.0, 0.0};
-  if (in) {
-    Grid<T> US{(const T*)C.us, C.nx}, VS{(const T*)C.vs, C.nx};
-    Grid<T> U{(const T*)C.u, C.nx}, V{(const T*)C.v, C.nx};
-    const double dc = (double)div_coherent<T>(US, VS, C.nx, C.ny, j, i, (T)C.dx, (T)C.dy);
-    const double cd = (double)div_central<T>(U, V, C.nx, C.ny, j, i, T(1), T(1));
-    v[0] = dc * dc;
-    v[1] = cd * cd;
-  }
-  double tot[2];
-  if (!reduce_arrive<2>(C, S, lt, v, 0u, tot)) return;
-  if (tid_flat() != 0) return;
-  S->div_after = sqrt(tot[0]);
-  S->continuity = sqrt(tot[1]) / pymax(1.0, (double)((size_t)C.nx * C.ny));
-  S->steps_done += 1;
-  S->steps_left -= 1;
-  if (C.mode == PF_MODE_STEP && S->steps_left > 0) {
-    S->phase = PH_STATS;
-  } else {
-    enter_done(S, active);
-  }
-}
-
-// One round: every tile executes its cavity's current phase. PH selects a
-// single phase at compile time (0 = dispatch on the phase read at entry).
-template <class T, int PH>
-__global__ __launch_bounds__(NTH) void k_round(const PfCav* __restrict__ cavs, PfScal* __restrict__ scal,
-                                               const int* __restrict__ tile_cav, int* __restrict__ active) {
-  const int t = blockIdx.x;
-  const int c = tile_cav[t];
-  if (c < 0) return;
-  const PfCav& C = cavs[c];
-  PfScal* S = scal + c;
-  const int phase = S->phase;
-  if (PH != 0 && phase != PH) return;
-  if (phase <= PH_IDLE || phase >= PH_DONE) return;
-  const int lt = t - C.tile0;
-  const int tx = lt % C.tiles_x, ty = lt / C.tiles_x;
-  const int i = tx * TX + threadIdx.x;
-  const int j = ty * TY + threadIdx.y;
-  const bool in = (i < C.nx) && (j < C.ny);
-  switch (phase) {
-    case PH_STATS: ph_stats<T>(C, S, lt, j, i, in); break;
-    case PH_PRED_A: ph_pred_a<T>(C, S, lt, j, i, in); break;
-    case PH_PRED_B: ph_pred_b<T>(C, S, lt, j, i, in); break;
-    case PH_PINIT: ph_pinit<T>(C, S, lt, j, i, in); break;
-    case PH_P: ph_p<T>(C, S, lt, j, i, in); break;
-    case PH_Q: ph_q<T>(C, S, lt, j, i, in); break;
-    case PH_OUTER_A: ph_outer_a<T>(C, S, lt, j, i, in); break;
-    case PH_OUTER_B: ph_outer_b<T>(C, S, lt, j, i, in); break;
-    case PH_FIN: ph_fin<T>(C, S, lt, j, i, in, active); break;
-    case PH_CORR_A: ph_corr_a<T>(C, S, lt, j, i, in); break;
-    case PH_CORR_B: ph_corr_b<T>(C, S, lt, j, i, in, active); break;
-    default: break;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Small utility kernels
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ void k_f64_to(const double* __restrict__ src, T* __restrict__ dst, size_t n) {
-  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) dst[k] = (T)src[k];
-}
-template <class T>
-__global__ void k_to_f64(const T* __restrict__ src, double* __restrict__ dst, size_t n) {
-  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) dst[k] = (double)src[k];
-}
-
-// Standalone float64 operators (numerics API), one cell per thread.
-__global__ void k_op(int opk, int nx, int ny, double dx, double dy, const double* __restrict__ in0,
-                     const double* __restrict__ in1, const double* __restrict__ in2, double* __restrict__ out0,
-                     double* __restrict__ out1) {
-  const int i = blockIdx.x * TX + threadIdx.x;
-  const int j = blockIdx.y * TY + threadIdx.y;
-  if (i >= nx || j >= ny) return;
-  const size_t k = (size_t)j * nx + i;
-  Grid<double> A{in0, nx}, B{in1, nx}, Cc{in2, nx};
-  switch (opk) {
-    case PF_OPK_GRAD_COHERENT:
-      out0[k] = grad_x_coherent<double>(A, nx, ny, j, i, dx);
-      out1[k] = grad_y_coherent<double>(A, nx, ny, j, i, dy);
-      break;
-    case PF_OPK_DIV_COHERENT: out0[k] = div_coherent<double>(A, B, nx, ny, j, i, dx, dy); break;
-    case PF_OPK_LAP_COHERENT: {
-      const double c = A(j, i);
-      const double e = (i + 1 < nx) ? A(j, i + 1) : 0.0;
-      const double w = (i > 0) ? A(j, i - 1) : 0.0;
-      const double n = (j + 1 < ny) ? A(j + 1, i) : 0.0;
-      const double s = (j > 0) ? A(j - 1, i) : 0.0;
-      out0[k] = lap_coherent5<double>(c, e, w, n, s, nx, ny, j, i, dx, dy);
-      break;
-    }
-    case PF_OPK_DIV_CENTRAL: out0[k] = div_central<double>(A, B, nx, ny, j, i, dx, dy); break;
-    case PF_OPK_ADV_QUICK: out0[k] = adv_quick<double>(A(j, i), B(j, i), Cc, nx, ny, j, i, dx, dy); break;
-    case PF_OPK_ADV_UPWIND: out0[k] = adv_upwind<double>(A(j, i), B(j, i), Cc, nx, ny, j, i, dx, dy); break;
-    case PF_OPK_NEG_LAP_PINNED: out0[k] = apply_op_at<double>(0, A, nx, ny, j, i, dx, dy); break;
-    case PF_OPK_POS_LAP_DIRICHLET: out0[k] = apply_op_at<double>(1, A, nx, ny, j, i, dx, dy); break;
-    default: break;
-  }
-}
-
-}  // namespace pf
 
 // ===========================================================================
 // Host side
@@ -688,23 +65,31 @@ struct pf_batch {
   int prec = 0;
   int max_cav = 0;
   int64_t max_tiles = 0;
+  int64_t round = 0;  // index of the next round; its parity selects the state buffers
+  int unroll = 2;     // tiles per unrolled chunk in the P/Q loops (PYFLOW_UNROLL: 1, 2, 4)
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<PfCav> hcav;       // slot -> description (nx == 0: free)
   std::vector<char> used;
   PfCav* dcav = nullptr;
   PfScal* dscal = nullptr;
-  int* dtile = nullptr;
+  int* dtile = nullptr;  // workgroup -> cavity slot
   int* dactive = nullptr;
   PfScal* hscal = nullptr;  // pinned mirror
   int* hactive = nullptr;   // pinned
-  int total_tiles = 0;
+  int total_blocks = 0;
+  int single_slot = -1;  // the only used slot when the batch holds one cavity, else -1
+  int max_tpb = 1;       // largest region height (tiles) over the batch's cavities
   bool tables_dirty = true;
   int64_t rounds = 0;
   int64_t predicted_rounds = 0;
   int64_t round_guard = 0;
   hipEvent_t ev = nullptr;
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;  // timing of the pending chunk
   bool chunk_pending = false;
+  bool timing = false;
+  double gpu_ms = 0.0;       // summed GPU time of launched chunks (timing on)
+  int64_t timed_rounds = 0;
 };
 
 static size_t esize(int prec) { return prec == PF_PREC_MIXED ? sizeof(float) : sizeof(double); }
@@ -726,15 +111,37 @@ static int check_params(const pf_cavity_params* p) {
   return PF_OK;
 }
 
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  if (!v || !*v) return dflt;
+  const int x = atoi(v);
+  return x > 0 ? x : dflt;
+}
+
+// Workgroup regions: M = tpb tiles (64 x 16 cells each) stacked in y, M a
+// power of two <= 8, starting from PYFLOW_TPB (default 4) and doubled while a
+// cavity would need more than PYFLOW_MAXBLK (default 1024) workgroups.
+static void blocking_of(int tiles_x, int tiles_y, int* nblk, int* tpb) {
+  int m = 1;
+  const int target = env_int("PYFLOW_TPB", 4);
+  while (m * 2 <= target && m < 8) m *= 2;
+  while (m > 1 && m / 2 >= tiles_y) m /= 2;  // no more rows than the cavity has
+  const int maxblk = env_int("PYFLOW_MAXBLK", 1024);
+  while (m < 8 && (int64_t)tiles_x * ((tiles_y + m - 1) / m) > maxblk) m *= 2;
+  *tpb = m;
+  *nblk = tiles_x * ((tiles_y + m - 1) / m);
+}
+
 static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   const size_t n = (size_t)p->nx * p->ny;
-  int tx, ty;
-  const size_t nt = tiles_of(p->nx, p->ny, &tx, &ty);
+  int tx, ty, nb, tpb;
+  tiles_of(p->nx, p->ny, &tx, &ty);
+  blocking_of(tx, ty, &nb, &tpb);
   size_t s = 0;
-  s += 9 * align256(n * esize(prec));                          // u v p us vs r d0 d1 x
+  s += 13 * align256(n * esize(prec));  // u v p us vs r d0 d1 x r1 w0 w1 pd
   s += align256(n * sizeof(double));                           // b
   if (prec == PF_PREC_MIXED) s += align256(n * sizeof(double));  // x64
-  s += align256(nt * NP * sizeof(double));                     // partials
+  s += align256((size_t)2 * nb * NP * sizeof(double));         // partials, two parities
   return s;
 }
 
@@ -744,6 +151,8 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   int tx, ty;
   c.ntiles = tiles_of(p->nx, p->ny, &tx, &ty);
   c.tiles_x = tx;
+  c.tiles_y = ty;
+  blocking_of(tx, ty, &c.nblk, &c.tpb);
   c.scheme = p->scheme;
   c.lid_on = p->lid_enabled ? 1 : 0;
   c.jacobi = p->jacobi ? 1 : 0;
@@ -764,16 +173,26 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
 
 static int sync_tables(pf_batch* b) {
   if (!b->tables_dirty) return PF_OK;
-  std::vector<int> tiles;
+  std::vector<int> tiles;  // workgroup -> cavity
   tiles.reserve(b->max_tiles);
   for (int s = 0; s < b->max_cav; ++s) {
     if (!b->used[s]) continue;
     PfCav& c = b->hcav[s];
-    c.tile0 = (int)tiles.size();
-    for (int t = 0; t < c.ntiles; ++t) tiles.push_back(s);
+    c.blk0 = (int)tiles.size();
+    for (int t = 0; t < c.nblk; ++t) tiles.push_back(s);
   }
-  if ((int64_t)tiles.size() > b->max_tiles) return fail(PF_EINVAL, "batch tile capacity exceeded");
-  b->total_tiles = (int)tiles.size();
+  if ((int64_t)tiles.size() > b->max_tiles) return fail(PF_EINVAL, "batch capacity exceeded");
+  b->total_blocks = (int)tiles.size();
+  int nused = 0, last = -1;
+  for (int s = 0; s < b->max_cav; ++s)
+    if (b->used[s]) {
+      ++nused;
+      last = s;
+    }
+  b->single_slot = (nused == 1) ? last : -1;
+  b->max_tpb = 1;
+  for (int s = 0; s < b->max_cav; ++s)
+    if (b->used[s]) b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
   HIPCHK(hipMemcpyAsync(b->dcav, b->hcav.data(), sizeof(PfCav) * b->max_cav, hipMemcpyHostToDevice, b->stream));
   if (!tiles.empty())
     HIPCHK(hipMemcpyAsync(b->dtile, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, b->stream));
@@ -782,14 +201,27 @@ static int sync_tables(pf_batch* b) {
   return PF_OK;
 }
 
+// Host edits of a cavity's state go to both parity buffers (the batch is idle).
 static int push_scal(pf_batch* b, int slot) {
-  HIPCHK(hipMemcpyAsync(b->dscal + slot, b->hscal + slot, sizeof(PfScal), hipMemcpyHostToDevice, b->stream));
+  for (int par = 0; par < 2; ++par)
+    HIPCHK(hipMemcpyAsync(b->dscal + (size_t)par * b->max_cav + slot, b->hscal + slot, sizeof(PfScal),
+                          hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PF_OK;
 }
 
+static int push_scal_all(pf_batch* b) {
+  for (int par = 0; par < 2; ++par)
+    HIPCHK(hipMemcpyAsync(b->dscal + (size_t)par * b->max_cav, b->hscal, sizeof(PfScal) * b->max_cav,
+                          hipMemcpyHostToDevice, b->stream));
+  return PF_OK;
+}
+
+// The latest state is in the buffer written by the last launched round.
 static int pull_scal(pf_batch* b) {
-  HIPCHK(hipMemcpyAsync(b->hscal, b->dscal, sizeof(PfScal) * b->max_cav, hipMemcpyDeviceToHost, b->stream));
+  const int par = (int)((b->round + 1) & 1);
+  HIPCHK(hipMemcpyAsync(b->hscal, b->dscal + (size_t)par * b->max_cav, sizeof(PfScal) * b->max_cav,
+                        hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PF_OK;
 }
@@ -801,11 +233,36 @@ static int check_slot(pf_batch* b, int slot) {
   return PF_OK;
 }
 
+template <class T, int U>
+static void launch_round_u(pf_batch* b, const KArgs& K) {
+  dim3 blk(TX, TY);
+  const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)(TY * b->max_tpb + 2) * LW * sizeof(float) : 0;
+  if (b->single_slot >= 0)
+    k_round<T, U, true><<<dim3(b->total_blocks), blk, lds, b->stream>>>(K);
+  else
+    k_round<T, U, false><<<dim3(b->total_blocks), blk, lds, b->stream>>>(K);
+}
+
 template <class T>
 static void launch_round(pf_batch* b) {
-  if (b->total_tiles <= 0) return;
-  dim3 blk(TX, TY);
-  k_round<T, 0><<<dim3(b->total_tiles), blk, 0, b->stream>>>(b->dcav, b->dscal, b->dtile, b->dactive);
+  if (b->total_blocks <= 0) return;
+  KArgs K;
+  std::memset(&K, 0, sizeof(K));
+  K.cavs = b->dcav;
+  K.scal = b->dscal;
+  K.blk_cav = b->dtile;
+  K.active = b->dactive;
+  K.max_cav = b->max_cav;
+  K.parity = (int)(b->round & 1);
+  K.slot0 = b->single_slot;
+  if (b->single_slot >= 0) K.cav0 = b->hcav[b->single_slot];
+  if (b->unroll >= 4)
+    launch_round_u<T, 4>(b, K);
+  else if (b->unroll == 2)
+    launch_round_u<T, 2>(b, K);
+  else
+    launch_round_u<T, 1>(b, K);
+  b->round += 1;
 }
 
 static void launch_rounds(pf_batch* b, int64_t n) {
@@ -844,6 +301,8 @@ static void fill_diag(const pf_batch* b, int slot, pf_step_diag* d) {
   d->nan_detected = s.nan;
   d->outer_passes = s.outer;
   d->steps_done = s.steps_done;
+  d->rp_iterations_run = s.it_run;
+  d->outer_passes_run = s.outer_run;
 }
 
 template <class T>
@@ -914,15 +373,18 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->max_tiles = max_tiles;
   b->stream = (hipStream_t)stream;
   b->hcav.assign(max_cav, PfCav{});
+  b->unroll = env_int("PYFLOW_UNROLL", 2);
   b->used.assign(max_cav, 0);
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dcav, sizeof(PfCav) * max_cav);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->dscal, sizeof(PfScal) * max_cav);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->dscal, 2 * sizeof(PfScal) * max_cav);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dtile, sizeof(int) * max_tiles);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dactive, sizeof(int));
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->hscal, sizeof(PfScal) * max_cav, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->hactive, sizeof(int), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev_t0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev_t1);
   if (e != hipSuccess) {
     std::string msg = std::string("allocation failed: ") + hipGetErrorString(e);
     pf_batch_destroy(b);
@@ -930,7 +392,7 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   }
   std::memset(b->hscal, 0, sizeof(PfScal) * max_cav);
   *b->hactive = 0;
-  e = hipMemsetAsync(b->dscal, 0, sizeof(PfScal) * max_cav, b->stream);
+  e = hipMemsetAsync(b->dscal, 0, 2 * sizeof(PfScal) * max_cav, b->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b->dtile, 0xff, sizeof(int) * max_tiles, b->stream);
   if (e == hipSuccess) e = hipMemsetAsync(b->dactive, 0, sizeof(int), b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -960,15 +422,15 @@ int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, s
   if (s < 0) return fail(PF_ENOMEM, "no free cavity slot");
   PfCav c{};
   fill_params(c, params, b->prec);
-  int64_t tiles_now = 0;
+  int64_t blocks_now = 0;
   for (int k = 0; k < b->max_cav; ++k)
-    if (b->used[k]) tiles_now += b->hcav[k].ntiles;
-  if (tiles_now + c.ntiles > b->max_tiles) return fail(PF_ENOMEM, "batch tile capacity exceeded");
+    if (b->used[k]) blocks_now += b->hcav[k].nblk;
+  if (blocks_now + c.nblk > b->max_tiles) return fail(PF_ENOMEM, "batch capacity exceeded");
   const size_t n = (size_t)c.nx * c.ny;
   const size_t vb = align256(n * esize(b->prec));
   char* w = (char*)workspace;
-  void** vecs[9] = {&c.u, &c.v, &c.p, &c.us, &c.vs, &c.r, &c.d0, &c.d1, &c.x};
-  for (int k = 0; k < 9; ++k) {
+  void** vecs[13] = {&c.u, &c.v, &c.p, &c.us, &c.vs, &c.r, &c.d0, &c.d1, &c.x, &c.r1, &c.w0, &c.w1, &c.pd};
+  for (int k = 0; k < 13; ++k) {
     *vecs[k] = w;
     w += vb;
   }
@@ -1050,6 +512,8 @@ int pf_batch_destroy(pf_batch* b) {
   if (b->hscal) (void)hipHostFree(b->hscal);
   if (b->hactive) (void)hipHostFree(b->hactive);
   if (b->ev) (void)hipEventDestroy(b->ev);
+  if (b->ev_t0) (void)hipEventDestroy(b->ev_t0);
+  if (b->ev_t1) (void)hipEventDestroy(b->ev_t1);
   delete b;
   return PF_OK;
 }
@@ -1173,13 +637,16 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
     PfScal& sc = b->hscal[s];
     if (!b->used[s]) continue;
     const bool on = !active || active[s];
-    sc.ticket = 0;
+    sc.pending = PH_IDLE;
+    sc.done_now = 0;
     if (!on) {
       sc.phase = PH_IDLE;
       continue;
     }
     const PfCav& c = b->hcav[s];
     sc.steps_done = 0;
+    sc.it_run = 0;
+    sc.outer_run = 0;
     if (c.mode == PF_MODE_STEP) {
       sc.phase = PH_STATS;
       sc.steps_left = nsteps;
@@ -1190,12 +657,13 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
       sc.phase = PH_PINIT;
       sc.steps_left = 1;
     }
-    const int64_t per = 4 * (int64_t)std::max(c.maxiter, 0) + 64;
+    const int64_t per = 4 * (int64_t)std::max(c.maxiter, 0) + 80;
     guard = std::max(guard, (int64_t)sc.steps_left * per + 64);
     ++nact;
   }
   *b->hactive = nact;
-  HIPCHK(hipMemcpyAsync(b->dscal, b->hscal, sizeof(PfScal) * b->max_cav, hipMemcpyHostToDevice, b->stream));
+  int rc2 = push_scal_all(b);
+  if (rc2) return rc2;
   HIPCHK(hipMemcpyAsync(b->dactive, b->hactive, sizeof(int), hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->rounds = 0;
@@ -1204,8 +672,13 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
 }
 
 static int chunk(pf_batch* b, int64_t n) {
+  if (b->timing) HIPCHK(hipEventRecord(b->ev_t0, b->stream));
   launch_rounds(b, n);
   HIPCHK(hipGetLastError());
+  if (b->timing) {
+    HIPCHK(hipEventRecord(b->ev_t1, b->stream));
+    b->timed_rounds += n;
+  }
   HIPCHK(hipMemcpyAsync(b->hactive, b->dactive, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipEventRecord(b->ev, b->stream));
   b->chunk_pending = true;
@@ -1223,6 +696,11 @@ int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* d
     while (true) {
       HIPCHK(hipEventSynchronize(b->ev));
       b->chunk_pending = false;
+      if (b->timing) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev_t0, b->ev_t1));
+        b->gpu_ms += ms;
+      }
       if (*b->hactive <= 0) break;
       if (b->rounds > b->round_guard) return fail(PF_ESTATE, "round limit exceeded (internal error)");
       rc = chunk(b, step);
@@ -1259,6 +737,11 @@ int pf_batch_poll(pf_batch* b, int* done) {
   }
   HIPCHK(q);
   b->chunk_pending = false;
+  if (b->timing) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev_t0, b->ev_t1));
+    b->gpu_ms += ms;
+  }
   if (*b->hactive <= 0) {
     int rc = pull_scal(b);
     if (rc) return rc;
@@ -1293,6 +776,21 @@ int pf_batch_centerline(pf_batch* b, int slot, double* out) {
   HIPCHK(hipStreamSynchronize(b->stream));
   for (int j = 0; j < c.ny; ++j)
     out[j] = (es == sizeof(float)) ? (double)((const float*)tmp.data())[j] : ((const double*)tmp.data())[j];
+  return PF_OK;
+}
+
+int pf_batch_set_timing(pf_batch* b, int enable) {
+  if (!b) return fail(PF_EINVAL, "batch is NULL");
+  b->timing = enable != 0;
+  b->gpu_ms = 0.0;
+  b->timed_rounds = 0;
+  return PF_OK;
+}
+
+int pf_batch_timing(pf_batch* b, double* gpu_ms, int64_t* rounds) {
+  if (!b || !gpu_ms || !rounds) return fail(PF_EINVAL, "bad arguments");
+  *gpu_ms = b->gpu_ms;
+  *rounds = b->timed_rounds;
   return PF_OK;
 }
 

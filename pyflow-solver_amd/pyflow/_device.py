@@ -105,7 +105,7 @@ class DeviceBatch:
 
     @staticmethod
     def tiles(nx: int, ny: int) -> int:
-        return ((nx + 63) // 64) * ((ny + 3) // 4)
+        return ((nx + 63) // 64) * ((ny + 15) // 16)
 
     def workspace_bytes(self, p: H.CavityParams) -> int:
         n = C.c_size_t()
@@ -219,6 +219,14 @@ class DeviceBatch:
         r = C.c_int64()
         H.check(self.L.pf_batch_rounds(self.handle, C.byref(r)), "pf_batch_rounds")
         return int(r.value)
+
+    def set_timing(self, enable: bool = True) -> None:
+        H.check(self.L.pf_batch_set_timing(self.handle, 1 if enable else 0), "pf_batch_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms, r = C.c_double(), C.c_int64()
+        H.check(self.L.pf_batch_timing(self.handle, C.byref(ms), C.byref(r)), "pf_batch_timing")
+        return float(ms.value), int(r.value)
 
     def close(self) -> None:
         if getattr(self, "handle", None):

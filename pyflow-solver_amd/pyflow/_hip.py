@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "pf_batch_remove", "pf_batch_set_params", "pf_batch_upload", "pf_batch_download",
     "pf_batch_zero_fields", "pf_batch_set_meta", "pf_batch_set_rhs", "pf_batch_get_solution",
     "pf_batch_set_dt", "pf_batch_run", "pf_batch_launch", "pf_batch_poll", "pf_batch_diag",
-    "pf_batch_centerline", "pf_batch_rounds", "pf_op_apply",
+    "pf_batch_centerline", "pf_batch_rounds", "pf_batch_set_timing", "pf_batch_timing", "pf_op_apply",
 )
 
 
@@ -65,7 +65,7 @@ class StepDiag(C.Structure):
         ("rp_residual", C.c_double), ("ru", C.c_double), ("rv", C.c_double),
         ("continuity", C.c_double), ("rp_info", C.c_int32), ("rp_iterations", C.c_int32),
         ("rp_converged", C.c_int32), ("nan_detected", C.c_int32), ("outer_passes", C.c_int32),
-        ("steps_done", C.c_int32),
+        ("steps_done", C.c_int32), ("rp_iterations_run", C.c_int32), ("outer_passes_run", C.c_int32),
     ]
 
     def history(self):
@@ -112,6 +112,8 @@ def _declare(L):
         "pf_batch_diag": ([P, P], I),
         "pf_batch_centerline": ([P, I, P], I),
         "pf_batch_rounds": ([P, C.POINTER(I64)], I),
+        "pf_batch_set_timing": ([P, I], I),
+        "pf_batch_timing": ([P, C.POINTER(D), C.POINTER(I64)], I),
         "pf_op_apply": ([I, I, I, D, D, P, P, P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
