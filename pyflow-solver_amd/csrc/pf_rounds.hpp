@@ -107,42 +107,80 @@ __device__ inline double redop(unsigned maxmask, int k, double a, double b) {
   return ((maxmask >> k) & 1u) ? nanmax(a, b) : a + b;
 }
 
-// Workgroup reduction of the first K of NP per-thread values: a fixed xor
-// butterfly inside each wave, then wave 0 loads the NWAVE wave totals into
-// lanes 0..NWAVE-1 and finishes with a second butterfly. The order of every
-// operation is fixed, so the result is bitwise reproducible (and identical in
-// every workgroup that reduces the same data). Result valid in thread 0.
-__device__ inline void block_reduce(double (&v)[NP], int K, unsigned maxmask, double (*sm)[NP]) {
+// Wave reduction on the DPP crossbar (no LDS): quad swaps, row half-mirror and
+// row mirror give 16-lane totals in every lane, then row_bcast15/31 fold the
+// four rows into lane 63. The combination order is fixed, so the result is
+// bitwise reproducible and identical in every workgroup reducing equal data.
+// Rows/lanes outside the DPP row mask read 0, the identity of both the sum and
+// the maximum of non-negative values used here.
+template <bool MAX>
+__device__ inline double red2(double a, double b) {
+  if constexpr (MAX)
+    return nanmax(a, b);
+  else
+    return a + b;
+}
+
+template <int CTRL, int ROWMASK, bool MAX>
+__device__ inline double dpp_step(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, ROWMASK, 0xF, false);
+  const long long ob = (long long)(unsigned)lo | ((long long)hi << 32);
+  return red2<MAX>(v, __longlong_as_double(ob));
+}
+
+template <bool MAX>
+__device__ inline double wave_reduce(double v) {
+  v = dpp_step<0xB1, 0xF, MAX>(v);   // quad_perm [1,0,3,2]
+  v = dpp_step<0x4E, 0xF, MAX>(v);   // quad_perm [2,3,0,1]
+  v = dpp_step<0x141, 0xF, MAX>(v);  // row_half_mirror
+  v = dpp_step<0x140, 0xF, MAX>(v);  // row_mirror
+  v = dpp_step<0x142, 0xA, MAX>(v);  // row_bcast15 into rows 1, 3
+  v = dpp_step<0x143, 0xC, MAX>(v);  // row_bcast31 into rows 2, 3
+  return v;                          // total in lane 63
+}
+
+// Workgroup reduction of the first K of NP per-thread values (bit k of MM set:
+// NaN-propagating maximum, else sum) into out[0..K) in shared memory. Ends
+// with a barrier, so every thread may read out[] afterwards.
+template <int K, unsigned MM>
+__device__ inline void block_reduce_k(double (&v)[NP], double (*sm)[NP], double* out) {
   const int tid = tid_flat();
   const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
+  for (int k = 0; k < K; ++k) v[k] = ((MM >> k) & 1u) ? wave_reduce<true>(v[k]) : wave_reduce<false>(v[k]);
+  if (lane == 63) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      if (k < K) {
-        const double o = __shfl_xor(v[k], off, 64);
-        v[k] = redop(maxmask, k, v[k], o);
-      }
-    }
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < NP; ++k) sm[wave][k] = v[k];
+    for (int k = 0; k < K; ++k) sm[wave][k] = v[k];
   }
   __syncthreads();
   if (wave == 0) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) v[k] = (lane < NWAVE && k < K) ? sm[lane][k] : 0.0;
-#pragma unroll
-    for (int off = NWAVE / 2; off >= 1; off >>= 1) {
-#pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        if (k < K) {
-          const double o = __shfl_xor(v[k], off, 64);
-          v[k] = redop(maxmask, k, v[k], o);
-        }
-      }
+    for (int k = 0; k < K; ++k) {
+      double t = (lane < NWAVE) ? sm[lane][k] : 0.0;
+      t = ((MM >> k) & 1u) ? wave_reduce<true>(t) : wave_reduce<false>(t);
+      if (lane == 63) out[k] = t;
     }
+  }
+  __syncthreads();
+}
+
+// Dispatch on a phase's partial layout (compile-time K and reduction ops).
+__device__ inline void block_reduce_phase(int ph, double (&v)[NP], double (*sm)[NP], double* out) {
+  switch (ph) {
+    case PH_STATS: block_reduce_k<3, 0x3u>(v, sm, out); break;
+    case PH_CORR_A: block_reduce_k<3, 0x4u>(v, sm, out); break;
+    case PH_PINIT:
+    case PH_OUTER_B:
+    case PH_CG0:
+    case PH_CG: block_reduce_k<3, 0u>(v, sm, out); break;
+    case PH_PRED_B:
+    case PH_Q:
+    case PH_CORR_B: block_reduce_k<2, 0u>(v, sm, out); break;
+    case PH_P:
+    case PH_FIN: block_reduce_k<1, 0u>(v, sm, out); break;
+    default: break;
   }
 }
 
@@ -960,12 +998,10 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
         for (int k = 0; k < NP; ++k)
           if (k < K) pa[k] = redop(mm, k, pa[k], prev[(size_t)q * NP + k]);
       }
-      block_reduce(pa, K, mm, sm);
-      if (tid == 0)
-        for (int k = 0; k < NP; ++k) tot[k] = pa[k];
+      block_reduce_phase(st.pending, pa, sm, tot);
     }
     if (tid == 0) tail(C, st, tot);
-    __syncthreads();  // tail result visible; sm free for reuse
+    __syncthreads();  // tail result visible; sm and tot free for reuse
   }
   const int phase = st.phase;
   if (phase <= PH_IDLE || phase >= PH_DONE) {
@@ -1011,11 +1047,8 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   }
   const int K = phase_nparts(phase);
   if (K > 0) {
-    block_reduce(acc, K, phase_maxmask(phase), sm);
-    if (tid == 0) {
-      double* mine = C.part + ((size_t)parity * C.nblk + b) * NP;
-      for (int k = 0; k < K; ++k) mine[k] = acc[k];
-    }
+    block_reduce_phase(phase, acc, sm, tot);
+    if (tid < K) C.part[((size_t)parity * C.nblk + b) * NP + tid] = tot[tid];
   }
   if (b == 0) {
     __syncthreads();

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 # golden case and quantity, how far the reference trajectory itself moves when
 # only its summation order changes (exactly rounded sums). The GPU must stay
 # within ENVELOPE_FACTOR times that, and u, v within the 1e-5 relative field
-# bound in every case.
+# bound -- or within twice the case's own envelope where the reference itself
+# moves by more than 5e-6 under a different summation order.
 ENVELOPE_FACTOR = 50.0
 FIELD_UV_CAP = 1e-5
 FLOOR_REL = 1e-12
@@ -93,7 +94,7 @@ def test_step_trajectory_matches_reference_fp64(golden_steps, name):
         assert maxrel <= max(FLOOR_REL, ENVELOPE_FACTOR * env[f"field_{f}_maxrel"]), (name, f, maxrel)
         assert l2rel <= max(FLOOR_REL, ENVELOPE_FACTOR * env[f"field_{f}_l2rel"]), (name, f, l2rel)
         if f in ("u", "v"):
-            assert maxrel <= FIELD_UV_CAP, (name, f, maxrel)
+            assert maxrel <= max(FIELD_UV_CAP, 2.0 * env[f"field_{f}_maxrel"]), (name, f, maxrel)
 
 
 def test_nan_propagation_matches_reference_fixture(golden_steps):
