@@ -37,10 +37,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024, help="grid size (n x n)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--inner-rtol", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the float64 comparison run")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="run only a plain 256 MiB device copy (10x) to calibrate PMC byte counters")
     return ap.parse_args()
 
 
@@ -122,6 +124,16 @@ def main():
     import torch
     from pyflow._device import DeviceBatch
     torch.cuda.set_device(0)
+    if args.copy_calibrate:
+        n_bytes = 256 << 20
+        src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibrate": {"bytes_read_per_copy": n_bytes, "bytes_written_per_copy": n_bytes,
+                                             "copies": 10}}))
+        return 0
     n = args.n
     N = n * n
     s = 4 if args.precision == "mixed" else 8

@@ -968,7 +968,14 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   const int max_cav = KA.max_cav;
   PfScal* scal = KA.scal;
   int* active = KA.active;
-  const int b = g - C.blk0;
+  // XCD-aware region order: workgroups are dispatched round-robin over the 8
+  // XCDs (g mod 8), each with its own L2. When the cavity's workgroup range is
+  // 8-aligned, XCD x gets the contiguous regions [x*nblk/8, (x+1)*nblk/8), so
+  // the halo rows/columns a region reads were mostly fetched into the same L2
+  // by its neighbours. Partials are indexed by region, so the reduction order
+  // does not depend on the mapping.
+  const int lg = g - C.blk0;
+  const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
   const int tid = tid_flat();
   const uint32_t* src = (const uint32_t*)(scal + (size_t)(parity ^ 1) * max_cav + c);
   uint32_t* dst = (uint32_t*)(scal + (size_t)parity * max_cav + c);
