@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024, help="grid size (n x n)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--inner-rtol", type=float, default=1e-5)
+    ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the float64 comparison run")
     ap.add_argument("--quiet", action="store_true")

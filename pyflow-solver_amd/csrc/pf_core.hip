@@ -236,7 +236,7 @@ static int check_slot(pf_batch* b, int slot) {
 template <class T, int U>
 static void launch_round_u(pf_batch* b, const KArgs& K) {
   dim3 blk(TX, TY);
-  const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)(TY * b->max_tpb + 2) * LW * sizeof(float) : 0;
+  const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)region_lds_floats(b->max_tpb) * sizeof(float) : 0;
   if (b->single_slot >= 0)
     k_round<T, U, true><<<dim3(b->total_blocks), blk, lds, b->stream>>>(K);
   else
@@ -364,9 +364,24 @@ int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* pa
   return PF_OK;
 }
 
+// The float32 CG region of 8 tiles stages 2 x 132 x 68 floats (71.8 KB) in
+// LDS, above the 64 KB a launch may request without opting in.
+static int allow_region_lds() {
+  const int bytes = region_lds_floats(8) * (int)sizeof(float);
+  const void* fns[6] = {(const void*)k_round<float, 1, true>, (const void*)k_round<float, 2, true>,
+                        (const void*)k_round<float, 4, true>, (const void*)k_round<float, 1, false>,
+                        (const void*)k_round<float, 2, false>, (const void*)k_round<float, 4, false>};
+  for (const void* f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  return PF_OK;
+}
+
 int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* stream, pf_batch** out) {
   if (!out || max_cav <= 0 || max_tiles <= 0) return fail(PF_EINVAL, "bad arguments");
   if (precision != PF_PREC_FP64 && precision != PF_PREC_MIXED) return fail(PF_EINVAL, "bad precision");
+  if (precision == PF_PREC_MIXED) {
+    const int rc = allow_region_lds();
+    if (rc) return rc;
+  }
   pf_batch* b = new pf_batch();
   b->prec = precision;
   b->max_cav = max_cav;

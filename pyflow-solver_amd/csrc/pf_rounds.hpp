@@ -29,7 +29,8 @@ constexpr int TX = 64;        // tile width = wavefront width
 constexpr int TY = 16;        // tile height; one wave per tile row
 constexpr int NTH = TX * TY;  // threads per workgroup
 constexpr int NWAVE = NTH / 64;
-constexpr int NP = 4;         // FP64 partial slots per workgroup
+constexpr int NP = 4;         // FP64 partial slots per workgroup (stride)
+constexpr int KMAX = 3;       // slots any phase uses
 
 enum Phase {
   PH_IDLE = 0,
@@ -141,47 +142,63 @@ __device__ inline double wave_reduce(double v) {
   return v;                          // total in lane 63
 }
 
-// Workgroup reduction of the first K of NP per-thread values (bit k of MM set:
-// NaN-propagating maximum, else sum) into out[0..K) in shared memory. Ends
-// with a barrier, so every thread may read out[] afterwards.
+// Wave reduction of the first K of NP values (bit k of MM set: NaN-propagating
+// maximum, else sum); the totals end in lane 63.
 template <int K, unsigned MM>
-__device__ inline void block_reduce_k(double (&v)[NP], double (*sm)[NP], double* out) {
-  const int tid = tid_flat();
-  const int lane = tid & 63, wave = tid >> 6;
+__device__ inline void wave_reduce_k(double (&v)[NP]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = ((MM >> k) & 1u) ? wave_reduce<true>(v[k]) : wave_reduce<false>(v[k]);
+}
+
+// Workgroup partials of the first K of NP per-thread values: every wave
+// reduces on the DPP crossbar, lane 63 parks the wave total in sm[wave], and
+// after one barrier thread k < K folds the NWAVE totals in wave order and
+// stores slot k to dst[k].
+template <int K, unsigned MM>
+__device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], double* dst) {
+  const int tid = tid_flat();
+  const int lane = tid & 63, wave = tid >> 6;
+  wave_reduce_k<K, MM>(v);
   if (lane == 63) {
 #pragma unroll
     for (int k = 0; k < K; ++k) sm[wave][k] = v[k];
   }
   __syncthreads();
-  if (wave == 0) {
+  if (tid < K) {
+    const bool mx = (MM >> tid) & 1u;
+    double t = sm[0][tid];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double t = (lane < NWAVE) ? sm[lane][k] : 0.0;
-      t = ((MM >> k) & 1u) ? wave_reduce<true>(t) : wave_reduce<false>(t);
-      if (lane == 63) out[k] = t;
-    }
+    for (int w = 1; w < NWAVE; ++w) t = mx ? nanmax(t, sm[w][tid]) : t + sm[w][tid];
+    dst[tid] = t;
   }
-  __syncthreads();
 }
 
 // Dispatch on a phase's partial layout (compile-time K and reduction ops).
-__device__ inline void block_reduce_phase(int ph, double (&v)[NP], double (*sm)[NP], double* out) {
-  switch (ph) {
-    case PH_STATS: block_reduce_k<3, 0x3u>(v, sm, out); break;
-    case PH_CORR_A: block_reduce_k<3, 0x4u>(v, sm, out); break;
-    case PH_PINIT:
-    case PH_OUTER_B:
-    case PH_CG0:
-    case PH_CG: block_reduce_k<3, 0u>(v, sm, out); break;
-    case PH_PRED_B:
-    case PH_Q:
-    case PH_CORR_B: block_reduce_k<2, 0u>(v, sm, out); break;
-    case PH_P:
-    case PH_FIN: block_reduce_k<1, 0u>(v, sm, out); break;
-    default: break;
+#define PF_PHASE_LAYOUT(F)                  \
+  switch (ph) {                             \
+    case PH_STATS: F(3, 0x3u); break;       \
+    case PH_CORR_A: F(3, 0x4u); break;      \
+    case PH_PINIT:                          \
+    case PH_OUTER_B:                        \
+    case PH_CG0:                            \
+    case PH_CG: F(3, 0u); break;            \
+    case PH_PRED_B:                         \
+    case PH_Q:                              \
+    case PH_CORR_B: F(2, 0u); break;        \
+    case PH_P:                              \
+    case PH_FIN: F(1, 0u); break;           \
+    default: break;                         \
   }
+
+__device__ inline void block_partials_phase(int ph, double (&v)[NP], double (*sm)[NP], double* dst) {
+#define PF_BP(K, MM) block_partials_k<K, MM>(v, sm, dst)
+  PF_PHASE_LAYOUT(PF_BP)
+#undef PF_BP
+}
+__device__ inline void wave_reduce_phase(int ph, double (&v)[NP]) {
+#define PF_WR(K, MM) wave_reduce_k<K, MM>(v)
+  PF_PHASE_LAYOUT(PF_WR)
+#undef PF_WR
 }
 
 // ---------------------------------------------------------------------------
@@ -208,7 +225,23 @@ __device__ inline void enter_done(PfScal& S) {
   S.done_now = 1;
 }
 
-__device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
+// Tail of a MIXED PCG iteration: beta = gamma'/gamma, alpha' = gamma'/(delta' -
+// beta gamma'/alpha). Pending phase PH_CG is the common case of a round.
+__device__ inline void tail_cg(const PfCav& C, PfScal& S, const double* tot) {
+  S.pending = PH_IDLE;
+  const double g1 = tot[0], d1 = tot[1];
+  const double beta = g1 / S.rho;
+  S.alpha = g1 / (d1 - beta * g1 / S.alpha);
+  S.beta = beta;
+  S.rho = g1;
+  S.rnorm2 = tot[2];
+  S.it += 1;
+  S.it_total += 1;
+  S.it_run += 1;
+  cg_next(C, S);
+}
+
+__attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
   const int ph = S.pending;
   S.pending = PH_IDLE;
   switch (ph) {
@@ -337,19 +370,7 @@ __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
         S.phase = (sqrt(S.rnorm2) < S.inner_atol) ? PH_OUTER_A : PH_CG;
       }
       break;
-    case PH_CG: {  // beta = gamma'/gamma, alpha' = gamma'/(delta' - beta gamma'/alpha)
-      const double g1 = tot[0], d1 = tot[1];
-      const double beta = g1 / S.rho;
-      S.alpha = g1 / (d1 - beta * g1 / S.alpha);
-      S.beta = beta;
-      S.rho = g1;
-      S.rnorm2 = tot[2];
-      S.it += 1;
-      S.it_total += 1;
-      S.it_run += 1;
-      cg_next(C, S);
-      break;
-    }
+    case PH_CG: tail_cg(C, S, tot); break;
     case PH_FIN: {
       const double rn = sqrt(tot[0]);
       S.rnorm_true = rn;
@@ -388,7 +409,7 @@ __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
 // Scalars a phase body reads, hoisted out of the shared state.
 struct PhaseArgs {
   double dt, mean, beta, alpha, x0;
-  int it, x_is_b;
+  int it, x_is_b, par;  // par = round parity
   float cx, cy, betaf, alphaf, minvf;  // float32 path: 1/dx^2, 1/dy^2, CG scalars, Jacobi weight
 };
 
@@ -447,7 +468,7 @@ __device__ inline void cell_pinit(const PfCav& C, const PhaseArgs& A, int j, int
     C.b[k] = bb;
   }
   const T rr = (T)bb;
-  ((T*)C.r)[k] = rr;
+  ((T*)((C.prec == PF_PREC_MIXED && A.par) ? C.r1 : C.r))[k] = rr;  // MIXED: r[q] of round q
   ((T*)C.x)[k] = T(0);
   if (C.prec == PF_PREC_MIXED) C.x64[k] = 0.0;
   const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rr;
@@ -635,13 +656,13 @@ __device__ inline void cell_outer_a(const PfCav& C, int j, int i) {
 }
 
 template <class T>
-__device__ inline void cell_outer_b(const PfCav& C, int j, int i, double* acc) {
+__device__ inline void cell_outer_b(const PfCav& C, const PhaseArgs& A, int j, int i, double* acc) {
   const size_t k = (size_t)j * C.nx + i;
   Grid<double> XG{C.x64, C.nx};
   const double ax = apply_op_at<double>(C.op, XG, C.nx, C.ny, j, i, C.dx, C.dy);
   const double r64 = C.b[k] - ax;
   const T rt = (T)r64;
-  ((T*)C.r)[k] = rt;
+  ((T*)(A.par ? C.r1 : C.r))[k] = rt;
   const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rt;
   acc[0] += r64 * r64;
   acc[1] += (double)rt * (double)z;
@@ -698,16 +719,21 @@ __device__ inline void cell_corr_b(const PfCav& C, int j, int i, double* acc) {
   acc[1] += cd * cd;
 }
 
-// MIXED inner PCG, single reduction per iteration (Chronopoulos & Gear):
-//   s_i = w_i + beta_i s_{i-1};  p_i = u_i + beta_i p_{i-1};  u = M r, w = A u
-//   x_{i+1} = x_i + alpha_i p_i; r_{i+1} = r_i - alpha_i s_i
-//   w_{i+1} = A M r_{i+1};  partials r.Mr, w.Mr, r.r of iteration i+1.
-// r_{i+1} is recomputed at the four neighbours from r_i, w_i, s_{i-1}, so r,
-// w and s are double-buffered by iteration parity; p and x are updated in place.
+// MIXED inner PCG, single reduction per iteration (Chronopoulos & Gear, with
+// s = A p applied explicitly to the stored search direction):
+//   u_k = M r_k;  p_k = u_k + beta_k p_{k-1};  s_k = A p_k
+//   x_{k+1} = x_k + alpha_k p_k;  r_{k+1} = r_k - alpha_k s_k
+//   u_{k+1} = M r_{k+1};  w_{k+1} = A u_{k+1}
+//   partials gamma = r_{k+1}.u_{k+1}, delta = w_{k+1}.u_{k+1}, |r_{k+1}|^2
+// and the tail sets beta_{k+1} = gamma'/gamma, alpha_{k+1} = gamma'/(delta' -
+// beta_{k+1} gamma'/alpha_k). Because the update of r uses A applied to the
+// same rounded p that updates x, the recurred residual stays consistent with
+// x up to the rounding of x itself. r and p are double-buffered by iteration
+// parity; x is updated in place. CG0 computes the partials of k = 0.
 template <class T>
 __device__ inline void cell_cg0(const PfCav& C, const PhaseArgs& A, int j, int i, double* acc) {
   const int nx = C.nx, ny = C.ny;
-  const T* R = (const T*)C.r;
+  const T* R = (const T*)(A.par ? C.r : C.r1);  // r[q^1]: written by PINIT / OUTER_B in round q-1
   auto U = [&](int jj, int ii) -> T {
     T m;
     if constexpr (sizeof(T) == sizeof(float)) m = minv_f32(C, A, jj, ii);
@@ -722,189 +748,192 @@ __device__ inline void cell_cg0(const PfCav& C, const PhaseArgs& A, int j, int i
   T wv;
   if constexpr (sizeof(T) == sizeof(float)) wv = apply_op5_fast(C.op, c, e, w, n, s, nx, ny, j, i, A.cx, A.cy);
   else wv = apply_op5<T>(C.op, c, e, w, n, s, nx, ny, j, i, (T)C.dx, (T)C.dy);
-  const size_t k = (size_t)j * nx + i;
-  ((T*)C.w0)[k] = wv;
-  const T rr = R[k];
+  const T rr = R[(size_t)j * nx + i];
   acc[0] += (double)rr * (double)c;
   acc[1] += (double)wv * (double)c;
   acc[2] += (double)rr * (double)rr;
 }
 
-template <class T>
-__device__ inline void cell_cg(const PfCav& C, const PhaseArgs& A, int j, int i, double* acc) {
-  const int nx = C.nx, ny = C.ny;
-  const int cur = A.it & 1;
-  const T* R = (const T*)(cur ? C.r1 : C.r);
-  const T* W = (const T*)(cur ? C.w1 : C.w0);
-  const T* Sold = (const T*)(cur ? C.d0 : C.d1);
-  T* Snew = (T*)(cur ? C.d1 : C.d0);
-  T* Rn = (T*)(cur ? C.r : C.r1);
-  T* Wn = (T*)(cur ? C.w0 : C.w1);
-  T* P = (T*)C.pd;
-  T* X = (T*)C.x;
-  const int first = (A.it == 0);
-  const T al = (T)A.alpha, be = (T)A.beta;
-  auto minv = [&](int jj, int ii) -> T {
-    if constexpr (sizeof(T) == sizeof(float)) return minv_f32(C, A, jj, ii);
-    else return (T)jacobi_minv(C.jacobi, C.invd, nx, ny, jj, ii);
-  };
-  // s_i and r_{i+1} at (jj, ii)
-  auto snew = [&](size_t kk) -> T { return first ? W[kk] : (T)(W[kk] + be * Sold[kk]); };
-  auto unext = [&](int jj, int ii, T* s_out) -> T {
-    const size_t kk = (size_t)jj * nx + ii;
-    const T sv = snew(kk);
-    if (s_out) *s_out = sv;
-    const T rv = R[kk] - al * sv;
-    return minv(jj, ii) * rv;
-  };
-  const size_t k = (size_t)j * nx + i;
-  T s_c;
-  const T c = unext(j, i, &s_c);
-  const T e = (i + 1 < nx) ? unext(j, i + 1, nullptr) : T(0);
-  const T w = (i > 0) ? unext(j, i - 1, nullptr) : T(0);
-  const T n = (j + 1 < ny) ? unext(j + 1, i, nullptr) : T(0);
-  const T s = (j > 0) ? unext(j - 1, i, nullptr) : T(0);
-  T wv;
-  if constexpr (sizeof(T) == sizeof(float)) wv = apply_op5_fast(C.op, c, e, w, n, s, nx, ny, j, i, A.cx, A.cy);
-  else wv = apply_op5<T>(C.op, c, e, w, n, s, nx, ny, j, i, (T)C.dx, (T)C.dy);
-  const T r_c = R[k] - al * s_c;
-  const T u_i = minv(j, i) * R[k];
-  const T p_i = first ? u_i : (T)(u_i + be * P[k]);
-  X[k] = X[k] + al * p_i;
-  P[k] = p_i;
-  Snew[k] = s_c;
-  Rn[k] = r_c;
-  Wn[k] = wv;
-  acc[0] += (double)r_c * (double)c;
-  acc[1] += (double)wv * (double)c;
-  acc[2] += (double)r_c * (double)r_c;
-}
+// One float32 PCG iteration over a workgroup's region (64 columns x RH = 16*M
+// rows), staged through LDS with a 2-cell halo:
+//   pass 1  (region_load, issued at kernel entry together with the state and
+//           partials loads, before the phase is known) r_k, p_{k-1}, x at the
+//           thread's M cells and r_k, p_{k-1} at (at most) one cell of the
+//           2-cell ring;
+//   pass 2  p_k into LDS (cells and ring), r_k of the ring into LDS; store
+//           p_k and x_{k+1};
+//   pass 3  s_k = A p_k, r_{k+1}, u_{k+1} at the cells and the 1-cell ring;
+//           store r_{k+1}; u_{k+1} into LDS;
+//   pass 4  w_{k+1} = A u_{k+1} and the partials at the cells.
+// HBM traffic per cell: read r, p, x; write r, p, x (the halo re-reads hit
+// L2). The r ring and u share one LDS array: a ring cell's r is read by the
+// thread that then writes the same cell's u, and no other thread touches ring
+// cells in pass 3.
+// Buffers follow the round parity q: the iteration run in round q reads r[q],
+// p[q^1] and writes r[q^1], p[q]; PINIT and OUTER_B (two rounds before the
+// first iteration of a solve) write r[q] and CG0 reads r[q^1].
+constexpr int HP = TX + 4;  // LDS row pitch: 64 columns + 2-cell halo each side
+__host__ __device__ constexpr int region_lds_floats(int m) { return 2 * (TY * m + 4) * HP; }
 
-// Float32 CG iteration over a workgroup's whole region (64 columns x 16*M
-// rows) in one pass. Every thread computes u_{i+1} = M r_{i+1} at its M cells
-// (rows 16*m + threadIdx.y) and threads 0..2*16*M+127 also at the 1-cell halo
-// (two rows, two columns) into LDS; after one barrier each thread applies the
-// stencil from LDS and stores s_i, r_{i+1}, w_{i+1}, p_i, x_{i+1}. All global
-// loads of a thread are independent and issued together.
-constexpr int LW = TX + 2;  // LDS row pitch: 64 columns + 1-cell halo each side
+__device__ inline float* mixed_r(const PfCav& C, int q) { return (float*)(q ? C.r1 : C.r); }
+__device__ inline float* mixed_p(const PfCav& C, int q) { return (float*)(q ? C.d1 : C.d0); }
+
 template <int M>
-__device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, double* acc, float* __restrict__ lu) {
+struct RegionIn {
+  float ro[M], po[M], xo[M];
+  float hr, hp;
+  int hrow, hcol;
+};
+
+template <int M>
+__device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M>& L) {
   const int nx = C.nx, ny = C.ny;
-  const int cur = A.it & 1;
-  const float* __restrict__ R = (const float*)(cur ? C.r1 : C.r);
-  const float* __restrict__ W = (const float*)(cur ? C.w1 : C.w0);
-  const float* __restrict__ Sold = (const float*)(cur ? C.d0 : C.d1);
-  float* __restrict__ Snew = (float*)(cur ? C.d1 : C.d0);
-  float* __restrict__ Rn = (float*)(cur ? C.r : C.r1);
-  float* __restrict__ Wn = (float*)(cur ? C.w0 : C.w1);
-  float* __restrict__ P = (float*)C.pd;
-  float* __restrict__ X = (float*)C.x;
-  const int first = (A.it == 0);
-  const float al = A.alphaf, be = A.betaf;
+  const float* __restrict__ R = mixed_r(C, par);
+  const float* __restrict__ Pold = mixed_p(C, par ^ 1);
+  const float* __restrict__ X = (const float*)C.x;
+  constexpr int RH = TY * M;
   const int li = threadIdx.x, lj = threadIdx.y;
   const int tid = lj * TX + li;
   const int i0 = (b % C.tiles_x) * TX;
-  const int j0 = (b / C.tiles_x) * (TY * M);
+  const int j0 = (b / C.tiles_x) * RH;
   const int i = i0 + li;
+  // Loads are unconditional (out-of-range lanes read cell 0 and discard it):
+  // loads under a branch make the compiler wait for all of them at the join,
+  // before the state has even been stored to LDS.
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    const bool ok = (i < nx) && (j < ny);
+    const size_t k = ok ? (size_t)j * nx + i : 0;
+    const float r = R[k], x = X[k], po = Pold[k];
+    L.ro[m] = ok ? r : 0.f;
+    L.xo[m] = ok ? x : 0.f;
+    L.po[m] = ok ? po : 0.f;
+  }
+  // 2-cell ring: rows 0,1 and RH+2,RH+3 (all HP columns), then columns 0,1
+  // and HP-2,HP-1 of rows 2..RH+1 (LDS coordinates row = j-j0+2, col = i-i0+2).
+  int hrow = -1, hcol = 0;
+  if (tid < 2 * HP) {
+    hrow = tid / HP;
+    hcol = tid % HP;
+  } else if (tid < 4 * HP) {
+    const int q = tid - 2 * HP;
+    hrow = RH + 2 + q / HP;
+    hcol = q % HP;
+  } else if (tid < 4 * HP + 4 * RH) {
+    const int q = tid - 4 * HP;
+    const int side = q / (2 * RH), rem = q % (2 * RH);
+    hrow = 2 + rem / 2;
+    hcol = side ? TX + 2 + (rem & 1) : (rem & 1);
+  }
+  L.hrow = hrow;
+  L.hcol = hcol;
+  const int hj = j0 + hrow - 2, hi = i0 + hcol - 2;
+  const bool hin = hrow >= 0 && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
+  const size_t kk = hin ? (size_t)hj * nx + hi : 0;
+  const float hr = R[kk], hp = Pold[kk];
+  L.hr = hin ? hr : 0.f;
+  L.hp = hin ? hp : 0.f;
+}
+
+template <int M>
+__device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, double* acc,
+                                     float* __restrict__ lds, const RegionIn<M>& L) {
+  const int nx = C.nx, ny = C.ny;
+  float* __restrict__ Rn = mixed_r(C, A.par ^ 1);
+  float* __restrict__ Pn = mixed_p(C, A.par);
+  float* __restrict__ X = (float*)C.x;
+  const int first = (A.it == 0);
+  const float al = A.alphaf, be = A.betaf;
   constexpr int RH = TY * M;
-  // Pass 1: issue every global load of this thread (M cells + at most one
-  // halo cell) before any store, so they are all in flight together.
-  float wv_[M], so_[M], ro_[M], po_[M], xo_[M];
+  constexpr int NR = RH + 4;
+  float* __restrict__ lp = lds;            // p_k on the region + 2-cell ring
+  float* __restrict__ lu = lds + NR * HP;  // r_k on the ring, then u_{k+1}
+  const int li = threadIdx.x, lj = threadIdx.y;
+  const int tid = lj * TX + li;
+  const int i0 = (b % C.tiles_x) * TX;
+  const int j0 = (b / C.tiles_x) * RH;
+  const int i = i0 + li;
+  // Pass 2.
   bool in[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = j0 + TY * m + lj;
     in[m] = (i < nx) && (j < ny);
-    wv_[m] = so_[m] = ro_[m] = po_[m] = xo_[m] = 0.f;
+    float pk = 0.f;
     if (in[m]) {
+      const float u = minv_f32(C, A, j, i) * L.ro[m];
+      pk = first ? u : (u + be * L.po[m]);
       const size_t k = (size_t)j * nx + i;
-      wv_[m] = W[k];
-      ro_[m] = R[k];
-      xo_[m] = X[k];
-      if (!first) {
-        so_[m] = Sold[k];
-        po_[m] = P[k];
-      }
+      // Streaming stores: the next round reads these lines from other XCDs'
+      // L2s or from memory anyway, so they are not kept dirty in this L2.
+      __builtin_nontemporal_store(pk, &Pn[k]);
+      __builtin_nontemporal_store(L.xo[m] + al * pk, &X[k]);
     }
+    lp[(TY * m + lj + 2) * HP + li + 2] = pk;
   }
-  // halo: rows j0-1 and j0+RH (threads 0..127), columns i0-1 and i0+64
-  // (threads 128..128+2*RH-1)
-  int hslot = -1, hj = 0, hi = 0;
-  if (tid < 2 * TX) {
-    const int side = tid / TX;
-    hi = i0 + (tid % TX);
-    hj = side ? j0 + RH : j0 - 1;
-    hslot = (side ? RH + 1 : 0) * LW + (tid % TX) + 1;
-  } else if (tid < 2 * TX + 2 * RH) {
-    const int q = tid - 2 * TX;
-    const int side = q / RH, row = q % RH;
-    hj = j0 + row;
-    hi = side ? i0 + TX : i0 - 1;
-    hslot = (row + 1) * LW + (side ? TX + 1 : 0);
-  }
-  const bool hin = hslot >= 0 && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
-  float hw = 0.f, hs = 0.f, hr = 0.f;
-  if (hin) {
-    const size_t kk = (size_t)hj * nx + hi;
-    hw = W[kk];
-    hr = R[kk];
-    if (!first) hs = Sold[kk];
-  }
-  // Pass 2: this thread's cells, then the halo value, into LDS.
-  float rc[M], uc[M];
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    rc[m] = uc[m] = 0.f;
-    const int j = j0 + TY * m + lj;
-    if (in[m]) {
-      const size_t k = (size_t)j * nx + i;
-      const float mi = minv_f32(C, A, j, i);
-      const float sv = first ? wv_[m] : (wv_[m] + be * so_[m]);
-      rc[m] = ro_[m] - al * sv;
-      uc[m] = mi * rc[m];
-      const float ui = mi * ro_[m];
-      const float p_i = first ? ui : (ui + be * po_[m]);
-      X[k] = xo_[m] + al * p_i;
-      P[k] = p_i;
-      Snew[k] = sv;
-      Rn[k] = rc[m];
-    }
-    lu[(TY * m + lj + 1) * LW + li + 1] = uc[m];
-  }
-  if (hslot >= 0) {
-    float hu = 0.f;
+  if (L.hrow >= 0) {
+    const int hj = j0 + L.hrow - 2, hi = i0 + L.hcol - 2;
+    const bool hin = hi >= 0 && hi < nx && hj >= 0 && hj < ny;
+    float p = 0.f;
     if (hin) {
-      const float sv = first ? hw : (hw + be * hs);
-      hu = minv_f32(C, A, hj, hi) * (hr - al * sv);
+      const float u = minv_f32(C, A, hj, hi) * L.hr;
+      p = first ? u : (u + be * L.hp);
     }
-    lu[hslot] = hu;
+    lp[L.hrow * HP + L.hcol] = p;
+    lu[L.hrow * HP + L.hcol] = hin ? L.hr : 0.f;
   }
   __syncthreads();
+  // Pass 3.
+  auto s_at = [&](int row, int col, int j, int ii) -> float {
+    const float* q = lp + row * HP + col;
+    return apply_op5_fast(C.op, q[0], q[1], q[-1], q[HP], q[-HP], nx, ny, j, ii, A.cx, A.cy);
+  };
+  float rk1[M], uk1[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    const int row = TY * m + lj + 2;
+    rk1[m] = uk1[m] = 0.f;
+    if (in[m]) {
+      const float sv = s_at(row, li + 2, j, i);
+      rk1[m] = L.ro[m] - al * sv;
+      uk1[m] = minv_f32(C, A, j, i) * rk1[m];
+      __builtin_nontemporal_store(rk1[m], &Rn[(size_t)j * nx + i]);
+    }
+    lu[row * HP + li + 2] = uk1[m];
+  }
+  // 1-cell ring: rows 1 and RH+2 (columns 1..TX+2), columns 1 and TX+2 (rows 2..RH+1)
+  int grow = -1, gcol = 0;
+  if (tid < 2 * (TX + 2)) {
+    grow = (tid < TX + 2) ? 1 : RH + 2;
+    gcol = 1 + tid % (TX + 2);
+  } else if (tid < 2 * (TX + 2) + 2 * RH) {
+    const int q = tid - 2 * (TX + 2);
+    grow = 2 + q % RH;
+    gcol = (q < RH) ? 1 : TX + 2;
+  }
+  if (grow >= 0) {
+    const int gj = j0 + grow - 2, gi = i0 + gcol - 2;
+    float u = 0.f;
+    if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+      const float sv = s_at(grow, gcol, gj, gi);
+      u = minv_f32(C, A, gj, gi) * (lu[grow * HP + gcol] - al * sv);
+    }
+    lu[grow * HP + gcol] = u;
+  }
+  __syncthreads();
+  // Pass 4.
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     if (in[m]) {
       const int j = j0 + TY * m + lj;
-      const int rr = TY * m + lj + 1;
-      const float c = uc[m];
-      const float e = lu[rr * LW + li + 2];
-      const float w = lu[rr * LW + li];
-      const float n = lu[(rr + 1) * LW + li + 1];
-      const float s = lu[(rr - 1) * LW + li + 1];
-      const float wv = apply_op5_fast(C.op, c, e, w, n, s, nx, ny, j, i, A.cx, A.cy);
-      Wn[(size_t)j * nx + i] = wv;
-      acc[0] += (double)rc[m] * (double)c;
-      acc[1] += (double)wv * (double)c;
-      acc[2] += (double)rc[m] * (double)rc[m];
+      const float* q = lu + (TY * m + lj + 2) * HP + li + 2;
+      const float wv = apply_op5_fast(C.op, q[0], q[1], q[-1], q[HP], q[-HP], nx, ny, j, i, A.cx, A.cy);
+      acc[0] += (double)rk1[m] * (double)uk1[m];
+      acc[1] += (double)wv * (double)uk1[m];
+      acc[2] += (double)rk1[m] * (double)rk1[m];
     }
-  }
-}
-
-__device__ inline void region_cg_f32_dispatch(const PfCav& C, const PhaseArgs& A, int b, double* acc, float* lu) {
-  switch (C.tpb) {
-    case 1: region_cg_f32<1>(C, A, b, acc, lu); break;
-    case 2: region_cg_f32<2>(C, A, b, acc, lu); break;
-    case 4: region_cg_f32<4>(C, A, b, acc, lu); break;
-    default: region_cg_f32<8>(C, A, b, acc, lu); break;
   }
 }
 
@@ -918,12 +947,11 @@ __device__ inline void cell_phase(int phase, const PfCav& C, const PhaseArgs& A,
     case PH_P: cell_p<T>(C, A, j, i, acc); break;
     case PH_Q: cell_q<T>(C, A, j, i, acc); break;
     case PH_OUTER_A: cell_outer_a<T>(C, j, i); break;
-    case PH_OUTER_B: cell_outer_b<T>(C, j, i, acc); break;
+    case PH_OUTER_B: cell_outer_b<T>(C, A, j, i, acc); break;
     case PH_FIN: cell_fin<T>(C, A, j, i, acc); break;
     case PH_CORR_A: cell_corr_a<T>(C, A, j, i, acc); break;
     case PH_CORR_B: cell_corr_b<T>(C, j, i, acc); break;
     case PH_CG0: cell_cg0<T>(C, A, j, i, acc); break;
-    case PH_CG: cell_cg<T>(C, A, j, i, acc); break;
     default: break;
   }
 }
@@ -946,12 +974,157 @@ struct KArgs {
   PfCav cav0;  // SINGLE: the cavity's descriptor
 };
 
+// The phases that run a few times per step, out of line: their code and
+// registers stay off the straight-line path of the PCG rounds.
+// Cg points at the cavity's descriptor in global memory (a kernel-argument
+// copy passed by reference would have to be spilled to scratch first).
+template <class T>
+__attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A,
+                                                     const TileIt ti, double (*sm)[NP], double* part_out) {
+  const PfCav& C = *Cg;
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0};
+  for (int ty = ti.ty; ty < ti.tyend; ++ty) {
+    const int i = ti.tx * TX + threadIdx.x;
+    const int j = ty * TY + threadIdx.y;
+    if (i < C.nx && j < C.ny) cell_phase<T>(phase, C, A, j, i, acc);
+  }
+  if (phase_nparts(phase) > 0) block_partials_phase(phase, acc, sm, part_out);
+}
+
+// Body of one round for workgroup b of cavity c. M > 0 (float32 only): the
+// cavity's region height in tiles; the region loads of a PCG iteration are
+// issued at entry, together with the state and partials loads, so the round
+// pays one memory round trip before its data is in registers. They are wasted
+// in the few rounds that run another phase.
+template <class T, int U, int M>
+__device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
+                                  double (*sm)[NP], double* tot, float* lds) {
+  const int parity = KA.parity;
+  const int tid = tid_flat();
+  const uint32_t* src = (const uint32_t*)(KA.scal + (size_t)(parity ^ 1) * KA.max_cav + c);
+  uint32_t* dst = (uint32_t*)(KA.scal + (size_t)parity * KA.max_cav + c);
+  constexpr int NW32 = (int)(sizeof(PfScal) / 4);
+  static_assert(NW32 <= NTH, "state copied with one word per thread");
+  // The state words and the previous round's partials (wave 0 only: lane l
+  // takes workgroups l, l+64, l+128, l+192) are loaded together, before
+  // anything is known about the phase. Loads are unconditional (out-of-range
+  // lanes read slot 0 and discard it), so the compiler can count them and let
+  // the region loads below stay in flight.
+  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
+  const int nblk = C.nblk;
+  const int lane = tid & 63;
+  const bool w0 = tid < 64;
+  constexpr int PRE = 4;
+  double pre[PRE][KMAX];
+#pragma unroll
+  for (int t = 0; t < PRE; ++t) {
+    const int q = lane + 64 * t;
+    const size_t qq = (w0 && q < nblk) ? (size_t)q : 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[qq * NP + k];
+  }
+  const uint32_t sw = src[tid < NW32 ? tid : 0];
+  RegionIn<(M > 0 ? M : 1)> L;
+  if constexpr (M > 0) region_load<M>(C, parity, b, L);
+  if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
+  __syncthreads();
+  if (st.pending != PH_IDLE) {
+    // Wave 0 folds the partials in a fixed order (per lane: increasing
+    // workgroup index; then the DPP tree), so every workgroup of the cavity
+    // computes the same bits; lane 63 runs the tail.
+    if (w0) {
+      // A PCG iteration is pending in nearly every round: its fold (sums
+      // only), reduction and tail form one straight-line block.
+      const bool hot = (M > 0) && st.pending == PH_CG;
+      const unsigned mm = hot ? 0u : phase_maxmask(st.pending);
+      double pa[NP] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < PRE; ++t)
+        if (lane + 64 * t < nblk) {
+#pragma unroll
+          for (int k = 0; k < KMAX; ++k) pa[k] = redop(mm, k, pa[k], pre[t][k]);
+        }
+      for (int q = lane + 64 * PRE; q < nblk; q += 64) {
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) pa[k] = redop(mm, k, pa[k], prev[(size_t)q * NP + k]);
+      }
+      if (__builtin_expect(hot, 1)) {
+        wave_reduce_k<3, 0u>(pa);
+        if (lane == 63) {
+#pragma unroll
+          for (int k = 0; k < NP; ++k) tot[k] = pa[k];
+          tail_cg(C, st, tot);
+        }
+      } else {
+        wave_reduce_phase(st.pending, pa);
+        if (lane == 63) {
+#pragma unroll
+          for (int k = 0; k < NP; ++k) tot[k] = pa[k];
+          tail(*Cg, st, tot);
+        }
+      }
+    }
+    __syncthreads();  // tail result visible
+  }
+  const int phase = st.phase;
+  if (phase <= PH_IDLE || phase >= PH_DONE) {
+    if (b == 0) {
+      const int newly_done = st.done_now;
+      __syncthreads();
+      if (tid == 0) st.done_now = 0;
+      __syncthreads();
+      if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+      if (tid == 0 && newly_done) atomicSub(KA.active, 1);
+    }
+    return;
+  }
+  PhaseArgs A;
+  A.dt = st.dt;
+  A.mean = st.mean;
+  A.beta = st.beta;
+  A.alpha = st.alpha;
+  A.it = st.it;
+  A.x_is_b = st.x_is_b;
+  A.par = parity;
+  A.x0 = 0.0;
+  if (phase == PH_FIN) A.x0 = st.x_is_b ? C.b[0] : C.x64[0];
+  A.cx = (float)(1.0 / (C.dx * C.dx));
+  A.cy = (float)(1.0 / (C.dy * C.dy));
+  A.betaf = (float)A.beta;
+  A.alphaf = (float)A.alpha;
+  A.minvf = (float)C.invd;
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0};
+  const int ty0 = (b / C.tiles_x) * C.tpb;
+  TileIt ti{b % C.tiles_x, ty0, min(ty0 + C.tpb, C.tiles_y)};
+  double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
+  if (__builtin_expect((M > 0) && phase == PH_CG, 1)) {
+    if constexpr (M > 0) region_cg_f32<M>(C, A, b, acc, lds, L);
+    block_partials_k<3, 0u>(acc, sm, part_out);
+  } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
+    if (phase == PH_P) {
+      for (int m = 0; m < C.tpb; m += U) chunk_p<T, U>(C, A, ti, acc);
+      block_partials_k<1, 0u>(acc, sm, part_out);
+    } else {
+      for (int m = 0; m < C.tpb; m += U) chunk_q<T, U>(C, A, ti, acc);
+      block_partials_k<2, 0u>(acc, sm, part_out);
+    }
+  } else {
+    round_cold<T>(phase, Cg, A, ti, sm, part_out);
+  }
+  if (b == 0) {
+    __syncthreads();
+    if (tid == 0) st.pending = phase;
+    __syncthreads();
+    if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+  }
+}
+
 template <class T, int U, bool SINGLE>
 __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   __shared__ PfScal st;
   __shared__ double sm[NWAVE][NP];
   __shared__ double tot[NP];
-  extern __shared__ float lds_dyn[];  // (16 * tpb + 2) * LW floats (float32 CG region)
+  extern __shared__ float lds_dyn[];  // region_lds_floats(tpb) floats (float32 CG region)
   const int g = blockIdx.x;
   int c;
   const PfCav* cp;
@@ -964,10 +1137,7 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
     cp = KA.cavs + c;
   }
   const PfCav& C = *cp;
-  const int parity = KA.parity;
-  const int max_cav = KA.max_cav;
-  PfScal* scal = KA.scal;
-  int* active = KA.active;
+  const PfCav* Cg = KA.cavs + c;  // the same descriptor in global memory
   // XCD-aware region order: workgroups are dispatched round-robin over the 8
   // XCDs (g mod 8), each with its own L2. When the cavity's workgroup range is
   // 8-aligned, XCD x gets the contiguous regions [x*nblk/8, (x+1)*nblk/8), so
@@ -976,92 +1146,15 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   // does not depend on the mapping.
   const int lg = g - C.blk0;
   const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
-  const int tid = tid_flat();
-  const uint32_t* src = (const uint32_t*)(scal + (size_t)(parity ^ 1) * max_cav + c);
-  uint32_t* dst = (uint32_t*)(scal + (size_t)parity * max_cav + c);
-  constexpr int NW32 = (int)(sizeof(PfScal) / 4);
-  // One memory round trip: the state words and every slot of the previous
-  // round's partials are loaded together (which slots matter is only known
-  // once the state has arrived).
-  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
-  double pa[NP] = {0.0, 0.0, 0.0, 0.0};
-  const int nblk = C.nblk;
-  double pq0[NP];
-  const bool have0 = tid < nblk;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) pq0[k] = have0 ? prev[(size_t)tid * NP + k] : 0.0;
-  for (int w = tid; w < NW32; w += NTH) ((uint32_t*)&st)[w] = src[w];
-  __syncthreads();
-  if (st.pending != PH_IDLE) {
-    const int K = phase_nparts(st.pending);
-    const unsigned mm = phase_maxmask(st.pending);
-    if (K > 0) {
-      if (have0) {
-#pragma unroll
-        for (int k = 0; k < NP; ++k) pa[k] = pq0[k];
-      }
-      for (int q = tid + NTH; q < nblk; q += NTH) {
-#pragma unroll
-        for (int k = 0; k < NP; ++k)
-          if (k < K) pa[k] = redop(mm, k, pa[k], prev[(size_t)q * NP + k]);
-      }
-      block_reduce_phase(st.pending, pa, sm, tot);
+  if constexpr (sizeof(T) == sizeof(float)) {
+    switch (C.tpb) {
+      case 1: round_body<T, U, 1>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      case 2: round_body<T, U, 2>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      case 4: round_body<T, U, 4>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      default: round_body<T, U, 8>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
     }
-    if (tid == 0) tail(C, st, tot);
-    __syncthreads();  // tail result visible; sm and tot free for reuse
-  }
-  const int phase = st.phase;
-  if (phase <= PH_IDLE || phase >= PH_DONE) {
-    if (b == 0) {
-      const int newly_done = st.done_now;
-      __syncthreads();
-      if (tid == 0) st.done_now = 0;
-      __syncthreads();
-      for (int w = tid; w < NW32; w += NTH) dst[w] = ((const uint32_t*)&st)[w];
-      if (tid == 0 && newly_done) atomicSub(active, 1);
-    }
-    return;
-  }
-  PhaseArgs A;
-  A.dt = st.dt;
-  A.mean = st.mean;
-  A.beta = st.beta;
-  A.alpha = st.alpha;
-  A.it = st.it;
-  A.x_is_b = st.x_is_b;
-  A.x0 = 0.0;
-  if (phase == PH_FIN) A.x0 = st.x_is_b ? C.b[0] : C.x64[0];
-  A.cx = (float)(1.0 / (C.dx * C.dx));
-  A.cy = (float)(1.0 / (C.dy * C.dy));
-  A.betaf = (float)A.beta;
-  A.alphaf = (float)A.alpha;
-  A.minvf = (float)C.invd;
-  double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  const int ty0 = (b / C.tiles_x) * C.tpb;
-  TileIt ti{b % C.tiles_x, ty0, min(ty0 + C.tpb, C.tiles_y)};
-  if (phase == PH_P) {
-    for (int m = 0; m < C.tpb; m += U) chunk_p<T, U>(C, A, ti, acc);
-  } else if (phase == PH_Q) {
-    for (int m = 0; m < C.tpb; m += U) chunk_q<T, U>(C, A, ti, acc);
-  } else if (sizeof(T) == sizeof(float) && phase == PH_CG) {
-    region_cg_f32_dispatch(C, A, b, acc, lds_dyn);
   } else {
-    for (int ty = ti.ty; ty < ti.tyend; ++ty) {
-      const int i = ti.tx * TX + threadIdx.x;
-      const int j = ty * TY + threadIdx.y;
-      if (i < C.nx && j < C.ny) cell_phase<T>(phase, C, A, j, i, acc);
-    }
-  }
-  const int K = phase_nparts(phase);
-  if (K > 0) {
-    block_reduce_phase(phase, acc, sm, tot);
-    if (tid < K) C.part[((size_t)parity * C.nblk + b) * NP + tid] = tot[tid];
-  }
-  if (b == 0) {
-    __syncthreads();
-    if (tid == 0) st.pending = phase;
-    __syncthreads();
-    for (int w = tid; w < NW32; w += NTH) dst[w] = ((const uint32_t*)&st)[w];
+    round_body<T, U, 0>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
   }
 }
 

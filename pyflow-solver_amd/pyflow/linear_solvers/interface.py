@@ -16,6 +16,7 @@ from typing import Literal
 import numpy as np
 
 from .. import _hip as H
+from .. import _device as D
 from .._device import DeviceCavity, cavity_params, precision_code
 
 __all__ = ["LinearSolveResult", "solve", "PinnedNegLaplacian", "DirichletPosLaplacian", "device_cg"]
@@ -76,13 +77,13 @@ _cache: dict = {}
 
 
 def device_cg(op: _PyflowOperator, b: np.ndarray, tol: float, maxiter: int, jacobi: bool,
-              precision=None, inner_rtol: float = 1e-4):
+              precision=None, inner_rtol: float = None):
     """SciPy-``cg``-equivalent PCG on the GPU. Returns (x, info, true_iters, ||b-Ax||, converged)."""
     prec = precision_code(precision)
     key = (op.nx, op.ny, prec)
     params = cavity_params(op.nx, op.ny, dx=op.dx, dy=op.dy, disable_advection=True, lin_tol=tol,
                            lin_maxiter=maxiter, jacobi=jacobi, op=op.op, mode=H.MODE_SOLVE,
-                           inner_rtol=inner_rtol)
+                           inner_rtol=D.DEFAULT_INNER_RTOL if inner_rtol is None else inner_rtol)
     cav = _cache.get(key)
     if cav is None:
         if len(_cache) > 8:

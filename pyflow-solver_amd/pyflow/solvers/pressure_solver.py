@@ -13,6 +13,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .. import _hip as H
+from .. import _device as D
 from .._device import cavity_params
 from .._mirror import config_precision, mirror_for
 from ..core.ghost_fields import State
@@ -69,13 +70,13 @@ def solve_pressure_poisson(state: State, dt: float, dx: float, dy: float, cfg, p
                                lin_tol=lin_tol, lin_maxiter=getattr(cfg, "lin_maxiter", 500),
                                jacobi=_jacobi_flag(preconditioner, cfg, nx, ny, dx, dy, False),
                                op=H.OP_POS_LAP_DIRICHLET, mode=H.MODE_SOLVE,
-                               inner_rtol=float(getattr(cfg, "inner_rtol", 1e-4)))
+                               inner_rtol=float(getattr(cfg, "inner_rtol", D.DEFAULT_INNER_RTOL)))
     else:
         params = cavity_params(nx, ny, dx=dx, dy=dy, disable_advection=True, lid_velocity=None,
                                lin_tol=lin_tol, lin_maxiter=getattr(cfg, "lin_maxiter", 400),
                                jacobi=_jacobi_flag(preconditioner, cfg, nx, ny, dx, dy, True),
                                op=H.OP_NEG_LAP_PINNED, mode=H.MODE_PROJECT,
-                               inner_rtol=float(getattr(cfg, "inner_rtol", 1e-4)))
+                               inner_rtol=float(getattr(cfg, "inner_rtol", D.DEFAULT_INNER_RTOL)))
     mirror = mirror_for(state, params, config_precision(cfg))
     mirror.before(params)
     if manufactured:

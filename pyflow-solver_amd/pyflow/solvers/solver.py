@@ -12,6 +12,7 @@ import math
 from typing import Any
 
 from .. import _hip as H
+from .. import _device as D
 from .._device import cavity_params
 from .._mirror import config_precision, mirror_for
 from ..core.ghost_fields import State
@@ -44,7 +45,7 @@ def step_params(config, nx: int, ny: int, dx: float, dy: float, mode: int = H.MO
         lin_maxiter=getattr(config, "lin_maxiter", 400),
         jacobi=bool(getattr(config, "enable_jacobi_pc", True)),
         mode=mode,
-        inner_rtol=float(getattr(config, "inner_rtol", 1e-4)),
+        inner_rtol=float(getattr(config, "inner_rtol", D.DEFAULT_INNER_RTOL)),
     )
 
 
