@@ -138,7 +138,7 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   tiles_of(p->nx, p->ny, &tx, &ty);
   blocking_of(tx, ty, &nb, &tpb);
   size_t s = 0;
-  s += 13 * align256(n * esize(prec));  // u v p us vs r d0 d1 x r1 w0 w1 pd
+  s += 10 * align256(n * esize(prec));  // u v p us vs r d0 d1 x r1
   s += align256(n * sizeof(double));                           // b
   if (prec == PF_PREC_MIXED) s += align256(n * sizeof(double));  // x64
   s += align256((size_t)2 * nb * NP * sizeof(double));         // partials, two parities
@@ -444,8 +444,8 @@ int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, s
   const size_t n = (size_t)c.nx * c.ny;
   const size_t vb = align256(n * esize(b->prec));
   char* w = (char*)workspace;
-  void** vecs[13] = {&c.u, &c.v, &c.p, &c.us, &c.vs, &c.r, &c.d0, &c.d1, &c.x, &c.r1, &c.w0, &c.w1, &c.pd};
-  for (int k = 0; k < 13; ++k) {
+  void** vecs[10] = {&c.u, &c.v, &c.p, &c.us, &c.vs, &c.r, &c.d0, &c.d1, &c.x, &c.r1};
+  for (int k = 0; k < 10; ++k) {
     *vecs[k] = w;
     w += vb;
   }

@@ -77,8 +77,7 @@ struct PfCav {
   int scheme, lid_on, jacobi, maxiter, op, mode, prec;
   double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
   void *u, *v, *p, *us, *vs, *r, *d0, *d1, *x;  // element type T
-  void *r1, *w0, *w1, *pd;                      // MIXED single-reduction PCG: r, w = A M r (two
-                                                // buffers each), search direction; s = A p in d0/d1
+  void* r1;                                     // MIXED: r double-buffered (r, r1); p in d0/d1
   double *b, *x64;                              // x64 aliases x in FP64 mode
   double* part;                                 // [2][nblk][NP]
 };

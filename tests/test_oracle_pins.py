@@ -193,3 +193,47 @@ def test_planted_mistakes_fail_the_pins(golden_ops, monkeypatch):
     X, Y = np.meshgrid(np.arange(1, 9.0), np.arange(1, 9.0))
     out = O.neg_laplacian_pinned(((X ** 2 + Y ** 2) / 4.0).reshape(-1), 8, 8, 1.0, 1.0).reshape(8, 8)
     assert not np.allclose(out[1:-1, 1:-1], -1.0)
+
+
+def test_planted_structural_mistakes_fail_the_pins(golden_ops, monkeypatch):
+    """A dropped term, a transposed operand and an off-by-one pin, each planted
+    in the oracle, are caught by the reference-written golden vectors."""
+    g = golden_ops
+    u, v, f = g["ops2_u"], g["ops2_v"], g["ops2_f"]
+    dx, dy = g["ops2_dxdy"]
+    # dropped term: the coherent divergence without its y part
+    orig_div = O.divergence_coherent
+
+    def div_no_y(uu, vv, a, b):
+        return orig_div(uu, np.zeros_like(vv), a, b)
+
+    monkeypatch.setattr(O, "divergence_coherent", div_no_y)
+    assert not np.array_equal(O.divergence_coherent(u, v, dx, dy), g["ops2_hdiv"])
+    monkeypatch.undo()
+    # transposed operand: the x derivative differenced along rows (axis 0)
+    # and the y derivative along columns
+    orig_grad = O.gradient_coherent
+
+    def grad_t(p, a, b):
+        gx_true, gy_true = orig_grad(p, a, b)
+        return gy_true, gx_true
+
+    monkeypatch.setattr(O, "gradient_coherent", grad_t)
+    gx, gy = O.gradient_coherent(f, dx, dy)
+    assert not np.array_equal(gx, g["ops2_gx"]) and not np.array_equal(gy, g["ops2_gy"])
+    monkeypatch.undo()
+    # off-by-one pin: the identity row at cell 1 instead of cell 0
+    orig_lap = O.laplacian_coherent
+
+    def pinned_at_1(x, nx, ny, a, b):
+        out = (-orig_lap(x.reshape(ny, nx), a, b)).reshape(-1)
+        out[1] = x[1]
+        return out
+
+    monkeypatch.setattr(O, "neg_laplacian_pinned", pinned_at_1)
+    nx, ny = 24, 20
+    uu = np.zeros((ny, nx)); vv = np.zeros((ny, nx)); pp = np.zeros((ny, nx))
+    uu[ny // 3, nx // 3] = 1.0
+    cfg = SimpleNamespace(lin_tol=1e-10, lin_maxiter=300, enable_jacobi_pc=True)
+    psol, _ = O.solve_pressure_poisson(uu, vv, pp, 0.01, 1.0, 1.0, cfg)
+    assert not np.array_equal(psol, g["ps_bump_pc_p"])
