@@ -43,6 +43,14 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a plain 256 MiB device copy (10x) to calibrate PMC byte counters")
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--step-scale", type=float, default=0.02,
+                    help="serve: scale of the per-request step count U{100..2000} (default 0.02: U{2..40})")
+    ap.add_argument("--workload", choices=["cavity", "poisson", "batch", "serve"], default="cavity",
+                    help="cavity: one lid-driven cavity (default, the headline metric); poisson: one 4096x4096 "
+                         "pinned-Neumann solve to 1e-10 (FP32 PCG + FP64 refinement); batch: 256 cavities of "
+                         "32..512 cells a side advanced together; serve: Poisson arrivals of cavity requests "
+                         "admitted into a running batch (continuous batching)")
     return ap.parse_args()
 
 
@@ -137,9 +145,7 @@ def main():
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); this build has no multi-GPU path. "
               "Independent cavities shard by request across GPUs; see DESIGN.md.", file=sys.stderr)
         return 2
-    import numpy as np
     import torch
-    from pyflow._device import DeviceBatch
     torch.cuda.set_device(0)
     if args.copy_calibrate:
         n_bytes = 256 << 20
@@ -151,6 +157,19 @@ def main():
         print(json.dumps({"copy_calibrate": {"bytes_read_per_copy": n_bytes, "bytes_written_per_copy": n_bytes,
                                              "copies": 10}}))
         return 0
+    if args.workload == "poisson":
+        return run_poisson(args)
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
+    return run_cavity(args)
+
+
+def run_cavity(args):
+    import numpy as np
+    import torch
+    from pyflow._device import DeviceBatch
     n = args.n
     N = n * n
     s = 4 if args.precision == "mixed" else 8
@@ -249,6 +268,271 @@ def main():
     }
     print(json.dumps(out), flush=True)
     batch.close()
+    return 0
+
+
+def run_logged(args, batch, nsteps, what):
+    """batch.run(nsteps) through launch/poll, logging the rounds executed every
+    10 s (the chunks double up to 256 rounds, so host polling gaps are a small
+    part of the wall time; GPU time comes from the chunks' events)."""
+    batch.launch(nsteps)
+    t0 = last = time.perf_counter()
+    while not batch.poll():
+        time.sleep(0.0002)
+        if time.perf_counter() - last > 10.0:
+            last = time.perf_counter()
+            log(args, f"[bench] {what}: {last - t0:.0f} s, {batch.rounds()} rounds")
+    return batch.diag()
+
+
+def gpu_roofline(kernel, alg_bytes, gpu_ms, rounds, traffic=None):
+    avg_launch_s = (gpu_ms / 1e3) / max(rounds, 1)
+    achieved = alg_bytes / max(rounds, 1) / max(avg_launch_s, 1e-30) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_launch_us": round(avg_launch_s * 1e6, 3), "alg_bytes_per_launch": int(alg_bytes / max(rounds, 1)),
+            "launches": rounds}
+
+
+def cpu_pcg_iteration_seconds(n, iters):
+    """FP64 NumPy oracle: seconds per Jacobi-PCG iteration on the n x n pinned
+    operator, from `iters` iterations on one BLAS thread."""
+    import numpy as np
+    from oracle import pyflow_oracle as O
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=1)
+    except Exception:  # pragma: no cover
+        limiter = None
+    b = np.random.default_rng(1).standard_normal(n * n)
+    b -= b.mean()
+    b[0] = 0.0
+    minv = O.jacobi_inv_diag(n, n, 1.0, 1.0)
+    t0 = time.perf_counter()
+    O.cg(lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0), b, 1e-30, iters, lambda r: minv * r)
+    dt = (time.perf_counter() - t0) / iters
+    if limiter is not None:
+        limiter.unregister()
+    return dt
+
+
+def run_poisson(args):
+    """One pinned-Neumann Poisson solve on a 4096 x 4096 grid: right-hand side
+    standard normal (seed 0), mean removed, cell 0 pinned to 0, x0 = 0; FP32
+    Jacobi-PCG passes with FP64 refinement until ||b - A x|| <= 1e-10 ||b||."""
+    import numpy as np
+    import torch
+    from pyflow import _hip as H
+    from pyflow._device import DeviceBatch, cavity_params
+    n = 4096 if args.n == 1024 else args.n
+    N = n * n
+    tol = 1e-10
+    rhs = np.random.default_rng(0).standard_normal((n, n))
+    rhs -= rhs.mean()
+    rhs[0, 0] = 0.0
+    params = cavity_params(n, n, dx=1.0, dy=1.0, lin_tol=tol, lin_maxiter=1_000_000, jacobi=True,
+                           op=H.OP_NEG_LAP_PINNED, mode=H.MODE_SOLVE, inner_rtol=args.inner_rtol)
+    batch = DeviceBatch([params], "mixed")
+    batch.set_rhs(0, rhs)
+    if args.warmup > 0:
+        log(args, f"[bench] warmup solve at {n}x{n}")
+        batch.run(1)
+    batch.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d = run_logged(args, batch, 1, "solve")[0]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms, rounds = batch.timing()
+    its, outer = int(d.rp_iterations), int(d.outer_passes)
+    rel = float(d.rp_residual) / float(d.bnorm)
+    log(args, f"[bench] {wall:.3f} s, {its} PCG iterations, {outer} refinement passes, ||r||/||b|| = {rel:.3e}")
+    if not (d.rp_converged and rel <= tol):
+        raise RuntimeError(f"solve did not reach {tol}: rel residual {rel:.3e}, info {d.rp_info}")
+    s = 4
+    alg_bytes = N * (its * pcg_iter_bytes(s) + outer * REFINE_BYTES)
+    pcg_gbs = N * its * pcg_iter_bytes(s) / (gpu_ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        m = 10
+        log(args, f"[bench] CPU baseline: {m} oracle PCG iterations at {n}x{n}")
+        t_it = cpu_pcg_iteration_seconds(n, m)
+        cpu = {"value": round(N * 8 * 8 / t_it / 1e9, 3), "unit": "GB/s (FP64 PCG, 8 vectors x 8 B per cell)",
+               "cores": 1, "kind": "port",
+               "sample": f"FP64 NumPy oracle PCG, {m} iterations at {n}x{n}: {t_it * 1e3:.1f} ms per iteration "
+                         f"(one BLAS thread); the GPU solve's {its} iterations would take {t_it * its:.0f} s"}
+    out = {
+        "metric": "PCG GB/s as % HBM peak (4096x4096 Poisson, FP32 PCG + FP64 refinement to 1e-10)",
+        "value": round(pcg_gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic: standard-normal right-hand side, seed 0",
+        "config": {"workload": f"pinned-Neumann Poisson {n}x{n}, Jacobi-PCG, tol 1e-10, x0 = 0", "grid": [n, n],
+                   "inner_rtol": args.inner_rtol},
+        "roofline": gpu_roofline("pf::k_round (one launch = one PCG iteration of the solve)", alg_bytes, gpu_ms,
+                                 rounds),
+        "pcg": {"iterations": its, "refinement_passes": outer, "rel_residual": rel,
+                "hbm_frac": round(pcg_gbs / HBM_PEAK_GBS, 4), "solve_s": round(wall, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    return 0
+
+
+def batch_params(count, seed, inner_rtol):
+    """256 cavities (seed 0): n_i = 32 * U{1..16}, Re_i log-uniform in [100, 5000]."""
+    import numpy as np
+    from pyflow._device import cavity_params
+    rng = np.random.default_rng(seed)
+    ns = 32 * rng.integers(1, 17, size=count)
+    res = np.exp(rng.uniform(np.log(100.0), np.log(5000.0), size=count))
+    params = [cavity_params(int(n), int(n), dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=float(re),
+                            cfl_target=0.5, cfl_growth=1.05, lin_tol=1e-6, lin_maxiter=10000, jacobi=True,
+                            inner_rtol=inner_rtol) for n, re in zip(ns, res)]
+    return params, ns
+
+
+def run_batch(args):
+    import numpy as np
+    import torch
+    from pyflow._device import DeviceBatch
+    steps = args.steps
+    params, ns = batch_params(256, 0, args.inner_rtol)
+    cells = int(np.sum(ns.astype(np.int64) ** 2))
+    batch = DeviceBatch(params, args.precision)
+    if args.warmup > 0:
+        log(args, f"[bench] warmup {args.warmup} steps, 256 cavities")
+        batch.run(args.warmup)
+        batch.zero_fields()
+        for k in range(len(params)):
+            batch.set_meta(k, 1e-2, [])
+    batch.set_timing(True)
+    log(args, f"[bench] timed run: {steps} steps of 256 cavities ({cells} cells)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    diags = run_logged(args, batch, steps, "batch")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms, rounds = batch.timing()
+    s = 4 if args.precision == "mixed" else 8
+    its = np.array([diags[k].rp_iterations_run for k in range(len(params))], dtype=np.int64)
+    outer = np.array([diags[k].outer_passes_run for k in range(len(params))], dtype=np.int64)
+    done = np.array([diags[k].steps_done for k in range(len(params))])
+    if not np.all(done == steps):
+        raise RuntimeError("not every cavity completed its steps")
+    n2 = ns.astype(np.int64) ** 2
+    alg_bytes = float(np.sum(n2 * (steps * step_other_bytes(s) + its * pcg_iter_bytes(s) + outer * REFINE_BYTES)))
+    value = cells * steps / wall / 1e6
+    nan = [k for k in range(len(params)) if diags[k].nan_detected]
+    log(args, f"[bench] {wall:.3f} s wall, {rounds} rounds, PCG iterations per cavity-step mean "
+              f"{its.mean() / steps:.0f}, max {its.max() / steps:.0f}")
+    out = {
+        "metric": "Mcell-steps/s, 256 independent cavities of 32..512 cells a side advanced together",
+        "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic: 256 lid-driven cavities from rest; n_i = 32*U{1..16}, Re_i log-uniform [100, 5000], "
+                "seed 0",
+        "config": {"workload": f"256 cavities, QUICK, tol 1e-6, lin_maxiter 10000, {steps} steps", "cells": cells,
+                   "precision": args.precision, "inner_rtol": args.inner_rtol},
+        "roofline": gpu_roofline("pf::k_round<float, U, false> (all cavities' regions in one launch)", alg_bytes,
+                                 gpu_ms, rounds),
+        "pcg": {"iterations_total": int(its.sum()), "mean_per_cavity_step": round(float(its.mean()) / steps, 1)},
+        "nan_cavities": nan,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    return 0
+
+
+def serve_requests(count, seed, step_scale):
+    """Requests (seed): n ~ round(exp(U(ln 64, ln 2048))), Re log-uniform in
+    [100, 1e4], steps ~ U{100..2000} times step_scale."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(64.0), np.log(2048.0), size=count))).astype(int)
+    res = np.exp(rng.uniform(np.log(100.0), np.log(1e4), size=count))
+    steps = np.maximum(1, np.rint(rng.integers(100, 2001, size=count) * step_scale)).astype(int)
+    return ns, res, steps
+
+
+def serve(reqs, arrivals, precision, inner_rtol, chunk_rounds=64, max_cav=64, log_fn=None):
+    """Continuous batching: admit arrived requests into the running batch,
+    advance it in chunks, retire finished cavities. Returns per-request
+    completion times (seconds from the start) and the makespan."""
+    import collections
+    from pyflow._device import DeviceBatch, cavity_params
+    ns, res, steps = reqs
+    srv = DeviceBatch(None, precision, max_cav=max_cav, max_tiles=1 << 20)
+    order = collections.deque(sorted(range(len(ns)), key=lambda k: arrivals[k]))
+    active, done = {}, {}
+    t0 = time.perf_counter()
+    last_log = t0
+    while order or active:
+        now = time.perf_counter() - t0
+        while order and arrivals[order[0]] <= now and len(active) < max_cav:
+            k = order.popleft()
+            p = cavity_params(int(ns[k]), int(ns[k]), dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0,
+                              Re=float(res[k]), cfl_target=0.5, cfl_growth=1.05, lin_tol=1e-6, lin_maxiter=10000,
+                              jacobi=True, inner_rtol=inner_rtol)
+            slot = srv.add(p)
+            srv.start(slot, int(steps[k]))
+            active[slot] = k
+        if not active:
+            time.sleep(max(0.0, arrivals[order[0]] - (time.perf_counter() - t0)))
+            continue
+        srv.advance(chunk_rounds)
+        while not srv.poll():
+            time.sleep(0.0001)
+        for slot in srv.finished():
+            k = active.pop(slot)
+            done[k] = time.perf_counter() - t0
+            srv.remove(slot)
+        if log_fn and time.perf_counter() - last_log > 10.0:
+            last_log = time.perf_counter()
+            log_fn(f"[bench] serving: {len(done)} done, {len(active)} running, {len(order)} waiting")
+    makespan = time.perf_counter() - t0
+    srv.close()
+    return done, makespan
+
+
+def run_serve(args):
+    import numpy as np
+    import torch  # noqa: F401
+    count = args.requests
+    reqs = serve_requests(count, 0, args.step_scale)
+    ns, res, steps = reqs
+    work = ns.astype(np.float64) ** 2 * steps
+    # capacity: the first requests served back to back (all arriving at t = 0)
+    m = min(8, count)
+    log(args, f"[bench] capacity probe: {m} requests at once")
+    cal = (ns[:m], res[:m], steps[:m])
+    _, span = serve(cal, np.zeros(m), args.precision, args.inner_rtol)
+    capacity = float(np.sum(work[:m])) / span  # cell-steps/s
+    lam = 0.7 * capacity / float(np.mean(work))  # requests/s at 70% of capacity
+    arrivals = np.cumsum(np.random.default_rng(1).exponential(1.0 / lam, size=count))
+    log(args, f"[bench] capacity {capacity / 1e6:.1f} Mcell-steps/s; arrival rate {lam:.3f}/s; {count} requests")
+    done, makespan = serve(reqs, arrivals, args.precision, args.inner_rtol, log_fn=lambda m_: log(args, m_))
+    lat = np.array([done[k] - arrivals[k] for k in range(count)])
+    out = {
+        "metric": "p50/p99 served-solve latency (continuous batching of lid-driven cavity requests)",
+        "value": round(float(np.percentile(lat, 50)), 4), "unit": "s (p50 latency)", "n_gpus": 1,
+        "steps": int(np.sum(steps)), "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": ("synthetic: Poisson arrivals (seed 1) at 70% of the measured capacity; n ~ round(exp(U(ln 64, "
+                 f"ln 2048))), Re log-uniform [100, 1e4], steps ~ U{{100..2000}} x {args.step_scale} (seed 0)"),
+        "config": {"workload": f"{count} cavity requests, QUICK, tol 1e-6, lin_maxiter 10000", "requests": count,
+                   "arrival_rate_per_s": round(lam, 4), "precision": args.precision},
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                      "mean": round(float(np.mean(lat)), 4), "max": round(float(np.max(lat)), 4)},
+        "throughput_mcell_steps_s": round(float(np.sum(work)) / makespan / 1e6, 3),
+        "capacity_mcell_steps_s": round(capacity / 1e6, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
     return 0
 
 

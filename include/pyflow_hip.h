@@ -130,7 +130,7 @@ int pf_batch_create(int precision, int ncav, const pf_cavity_params* params,
 int pf_batch_destroy(pf_batch* b);
 
 /* Dynamic batches (continuous batching): an empty batch with room for
- * `max_cav` cavities and `max_tiles` 64x16-cell tiles in total; cavities are
+ * `max_cav` cavities and `max_tiles` workgroup regions in total; cavities are
  * added with their own workspace (pf_batch_workspace_bytes for ncav = 1) and
  * removed between runs. Slots are the indices used by every per-cavity call. */
 int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* stream, pf_batch** out);
@@ -169,6 +169,22 @@ int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* d
 int pf_batch_launch(pf_batch* b, int nsteps, const int32_t* active);
 int pf_batch_poll(pf_batch* b, int* done);
 int pf_batch_diag(pf_batch* b, pf_step_diag* diag);
+
+/* Continuous batching (served solves; no counterpart in the reference, whose
+ * driver advances one cavity per process, drivers/simulation_driver.py:58-110).
+ * Between chunks of rounds:
+ *   pf_batch_start    arms cavity `slot` for `nsteps` steps (one projection or
+ *                     solve in the other modes); the other cavities keep their
+ *                     device-resident state and continue where they are;
+ *   pf_batch_advance  queues `rounds` rounds over the batch and returns at once;
+ *                     pf_batch_poll() then reports 1 when they have run and
+ *                     queues nothing more;
+ *   pf_batch_finished writes finished[slot] = 1 for every cavity that completed
+ *                     its work (then pf_batch_diag has its results and the slot
+ *                     may be removed). `finished` has max_cav entries. */
+int pf_batch_start(pf_batch* b, int slot, int nsteps);
+int pf_batch_advance(pf_batch* b, int64_t rounds);
+int pf_batch_finished(pf_batch* b, int32_t* finished);
 
 /* u[:, nx//2] of one cavity as float64 (drivers/simulation_driver.py:90-96). */
 int pf_batch_centerline(pf_batch* b, int cav, double* out);

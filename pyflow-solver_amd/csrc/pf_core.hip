@@ -87,6 +87,8 @@ struct pf_batch {
   hipEvent_t ev = nullptr;
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;  // timing of the pending chunk
   bool chunk_pending = false;
+  bool serving = false;  // chunks queued by pf_batch_advance: poll queues nothing more
+  int64_t poll_chunk = 16;  // rounds queued by the next pf_batch_poll (doubles up to 256)
   bool timing = false;
   double gpu_ms = 0.0;       // summed GPU time of launched chunks (timing on)
   int64_t timed_rounds = 0;
@@ -169,6 +171,14 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   c.tol = p->lin_tol;
   c.invd = p->inv_diag;
   c.inner_rtol = (p->inner_rtol > 0.0) ? p->inner_rtol : 1e-4;
+  // MIXED: accumulate the PCG iterate in FP64 on large grids. Rounding the
+  // iterate to FP32 perturbs the true residual by about eps32 * cond(A) of the
+  // pass's starting residual; cond(A) ~ (2n/pi)^2 makes that 0.1 at n = 2048
+  // and 0.4 at n = 4096, where the refinement passes stop contracting.
+  // PYFLOW_XD=0/1 forces it off/on.
+  const int xd_env = env_int("PYFLOW_XD", -1);
+  const int nmax = std::max(p->nx, p->ny);
+  c.xd = (prec == PF_PREC_MIXED) ? (xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0)) : 0;
 }
 
 static int sync_tables(pf_batch* b) {
@@ -701,6 +711,7 @@ static int chunk(pf_batch* b, int64_t n) {
 }
 
 int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* diag) {
+  if (b) b->serving = false;
   int rc = arm(b, nsteps, active);
   if (rc) return rc;
   if (*b->hactive > 0) {
@@ -733,9 +744,11 @@ int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* d
 }
 
 int pf_batch_launch(pf_batch* b, int nsteps, const int32_t* active) {
+  if (b) b->serving = false;
   int rc = arm(b, nsteps, active);
   if (rc) return rc;
   if (*b->hactive <= 0) return PF_OK;
+  b->poll_chunk = 16;
   return chunk(b, 16);
 }
 
@@ -757,6 +770,10 @@ int pf_batch_poll(pf_batch* b, int* done) {
     HIPCHK(hipEventElapsedTime(&ms, b->ev_t0, b->ev_t1));
     b->gpu_ms += ms;
   }
+  if (b->serving) {
+    *done = 1;
+    return PF_OK;
+  }
   if (*b->hactive <= 0) {
     int rc = pull_scal(b);
     if (rc) return rc;
@@ -764,9 +781,63 @@ int pf_batch_poll(pf_batch* b, int* done) {
     return PF_OK;
   }
   if (b->rounds > b->round_guard) return fail(PF_ESTATE, "round limit exceeded (internal error)");
-  int rc = chunk(b, 32);
+  b->poll_chunk = std::min<int64_t>(b->poll_chunk * 2, 256);
+  int rc = chunk(b, b->poll_chunk);
   if (rc) return rc;
   *done = 0;
+  return PF_OK;
+}
+
+// Continuous batching: one cavity is armed while the others keep their
+// device-resident state; its phase machine starts with the next round.
+int pf_batch_start(pf_batch* b, int slot, int nsteps) {
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  if (nsteps <= 0) return fail(PF_EINVAL, "nsteps must be > 0");
+  const int par = (int)((b->round + 1) & 1);  // buffer of the last round
+  HIPCHK(hipMemcpyAsync(b->hscal + slot, b->dscal + (size_t)par * b->max_cav + slot, sizeof(PfScal),
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  PfScal& sc = b->hscal[slot];
+  if (sc.phase != PH_IDLE && sc.phase != PH_DONE) return fail(PF_ESTATE, "cavity is already running");
+  const PfCav& c = b->hcav[slot];
+  sc.pending = PH_IDLE;
+  sc.done_now = 0;
+  sc.steps_done = 0;
+  sc.it_run = 0;
+  sc.outer_run = 0;
+  if (c.mode == PF_MODE_STEP) {
+    sc.phase = PH_STATS;
+    sc.steps_left = nsteps;
+  } else {
+    sc.phase = (c.mode == PF_MODE_PROJECT) ? PH_PRED_A : PH_PINIT;
+    sc.steps_left = 1;
+  }
+  rc = push_scal(b, slot);
+  if (rc) return rc;
+  *b->hactive += 1;
+  HIPCHK(hipMemcpyAsync(b->dactive, b->hactive, sizeof(int), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->serving = true;
+  return PF_OK;
+}
+
+int pf_batch_advance(pf_batch* b, int64_t rounds) {
+  if (!b || rounds <= 0) return fail(PF_EINVAL, "bad arguments");
+  if (b->chunk_pending) return fail(PF_ESTATE, "batch is running");
+  int rc = sync_tables(b);
+  if (rc) return rc;
+  b->serving = true;
+  if (b->total_blocks <= 0) return PF_OK;
+  return chunk(b, rounds);
+}
+
+int pf_batch_finished(pf_batch* b, int32_t* finished) {
+  if (!b || !finished) return fail(PF_EINVAL, "bad arguments");
+  if (b->chunk_pending) return fail(PF_ESTATE, "batch is running");
+  int rc = pull_scal(b);
+  if (rc) return rc;
+  for (int s = 0; s < b->max_cav; ++s) finished[s] = (b->used[s] && b->hscal[s].phase == PH_DONE) ? 1 : 0;
   return PF_OK;
 }
 

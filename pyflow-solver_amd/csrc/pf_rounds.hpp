@@ -20,6 +20,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pf_cells.hpp"
 #include "pyflow_hip.h"
 
@@ -75,6 +77,7 @@ __device__ __host__ inline unsigned phase_maxmask(int ph) {
 struct PfCav {
   int nx, ny, tiles_x, tiles_y, ntiles, nblk, blk0, tpb;  // tpb = M: tiles per workgroup, stacked in y
   int scheme, lid_on, jacobi, maxiter, op, mode, prec;
+  int xd;  // MIXED: the PCG iterate is accumulated in FP64 (x64) instead of FP32
   double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
   void *u, *v, *p, *us, *vs, *r, *d0, *d1, *x;  // element type T
   void* r1;                                     // MIXED: r double-buffered (r, r1); p in d0/d1
@@ -204,16 +207,20 @@ __device__ inline void wave_reduce_phase(int ph, double (&v)[NP]) {
 // Scalar tails: run by one thread on the workgroup's shared copy of the state
 // after the pending phase's partials have been summed into tot[].
 // ---------------------------------------------------------------------------
+// First phase of a refinement pass: with an FP64 iterate there is no FP32
+// correction to fold in, so the pass starts with the FP64 residual.
+__device__ inline int outer_entry(const PfCav& C) { return C.xd ? PH_OUTER_B : PH_OUTER_A; }
+
 __device__ inline void cg_next(const PfCav& C, PfScal& S) {
   const bool mixed = (C.prec == PF_PREC_MIXED);
   if (S.it_total >= C.maxiter) {
     S.info = C.maxiter;
-    S.phase = mixed ? PH_OUTER_A : PH_FIN;
+    S.phase = mixed ? outer_entry(C) : PH_FIN;
     return;
   }
   if (sqrt(S.rnorm2) < S.inner_atol) {
     if (!mixed) S.info = 0;
-    S.phase = mixed ? PH_OUTER_A : PH_FIN;
+    S.phase = mixed ? outer_entry(C) : PH_FIN;
     return;
   }
   S.phase = mixed ? PH_CG : PH_P;
@@ -364,9 +371,9 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
       S.it = 0;
       if (S.it_total >= C.maxiter) {
         S.info = C.maxiter;
-        S.phase = PH_OUTER_A;
+        S.phase = outer_entry(C);
       } else {
-        S.phase = (sqrt(S.rnorm2) < S.inner_atol) ? PH_OUTER_A : PH_CG;
+        S.phase = (sqrt(S.rnorm2) < S.inner_atol) ? outer_entry(C) : PH_CG;
       }
       break;
     case PH_CG: tail_cg(C, S, tot); break;
@@ -777,15 +784,16 @@ __host__ __device__ constexpr int region_lds_floats(int m) { return 2 * (TY * m 
 __device__ inline float* mixed_r(const PfCav& C, int q) { return (float*)(q ? C.r1 : C.r); }
 __device__ inline float* mixed_p(const PfCav& C, int q) { return (float*)(q ? C.d1 : C.d0); }
 
-template <int M>
+template <int M, bool XD>
 struct RegionIn {
-  float ro[M], po[M], xo[M];
+  float ro[M], po[M];
+  std::conditional_t<XD, double, float> xo[M];  // iterate: x64 (XD) or the FP32 correction
   float hr, hp;
   int hrow, hcol;
 };
 
-template <int M>
-__device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M>& L) {
+template <int M, bool XD>
+__device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M, XD>& L) {
   const int nx = C.nx, ny = C.ny;
   const float* __restrict__ R = mixed_r(C, par);
   const float* __restrict__ Pold = mixed_p(C, par ^ 1);
@@ -804,10 +812,16 @@ __device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M>& 
     const int j = j0 + TY * m + lj;
     const bool ok = (i < nx) && (j < ny);
     const size_t k = ok ? (size_t)j * nx + i : 0;
-    const float r = R[k], x = X[k], po = Pold[k];
+    const float r = R[k], po = Pold[k];
     L.ro[m] = ok ? r : 0.f;
-    L.xo[m] = ok ? x : 0.f;
     L.po[m] = ok ? po : 0.f;
+    if constexpr (XD) {
+      const double x = C.x64[k];
+      L.xo[m] = ok ? x : 0.0;
+    } else {
+      const float x = X[k];
+      L.xo[m] = ok ? x : 0.f;
+    }
   }
   // 2-cell ring: rows 0,1 and RH+2,RH+3 (all HP columns), then columns 0,1
   // and HP-2,HP-1 of rows 2..RH+1 (LDS coordinates row = j-j0+2, col = i-i0+2).
@@ -835,9 +849,9 @@ __device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M>& 
   L.hp = hin ? hp : 0.f;
 }
 
-template <int M>
+template <int M, bool XD>
 __device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, double* acc,
-                                     float* __restrict__ lds, const RegionIn<M>& L) {
+                                     float* __restrict__ lds, const RegionIn<M, XD>& L) {
   const int nx = C.nx, ny = C.ny;
   float* __restrict__ Rn = mixed_r(C, A.par ^ 1);
   float* __restrict__ Pn = mixed_p(C, A.par);
@@ -864,10 +878,13 @@ __device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, 
       const float u = minv_f32(C, A, j, i) * L.ro[m];
       pk = first ? u : (u + be * L.po[m]);
       const size_t k = (size_t)j * nx + i;
-      // Streaming stores: the next round reads these lines from other XCDs'
-      // L2s or from memory anyway, so they are not kept dirty in this L2.
-      __builtin_nontemporal_store(pk, &Pn[k]);
-      __builtin_nontemporal_store(L.xo[m] + al * pk, &X[k]);
+      // Ordinary stores. Non-temporal (streaming) stores here let the next
+      // round's entry loads see stale values in multi-cavity batches.
+      Pn[k] = pk;
+      if constexpr (XD)
+        C.x64[k] = L.xo[m] + A.alpha * (double)pk;
+      else
+        X[k] = L.xo[m] + al * pk;
     }
     lp[(TY * m + lj + 2) * HP + li + 2] = pk;
   }
@@ -898,7 +915,7 @@ __device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, 
       const float sv = s_at(row, li + 2, j, i);
       rk1[m] = L.ro[m] - al * sv;
       uk1[m] = minv_f32(C, A, j, i) * rk1[m];
-      __builtin_nontemporal_store(rk1[m], &Rn[(size_t)j * nx + i]);
+      Rn[(size_t)j * nx + i] = rk1[m];
     }
     lu[row * HP + li + 2] = uk1[m];
   }
@@ -995,7 +1012,7 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
 // issued at entry, together with the state and partials loads, so the round
 // pays one memory round trip before its data is in registers. They are wasted
 // in the few rounds that run another phase.
-template <class T, int U, int M>
+template <class T, int U, int M, bool XD>
 __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
                                   double (*sm)[NP], double* tot, float* lds) {
   const int parity = KA.parity;
@@ -1023,8 +1040,8 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
     for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[qq * NP + k];
   }
   const uint32_t sw = src[tid < NW32 ? tid : 0];
-  RegionIn<(M > 0 ? M : 1)> L;
-  if constexpr (M > 0) region_load<M>(C, parity, b, L);
+  RegionIn<(M > 0 ? M : 1), XD> L;
+  if constexpr (M > 0) region_load<M, XD>(C, parity, b, L);
   if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
   __syncthreads();
   if (st.pending != PH_IDLE) {
@@ -1097,7 +1114,7 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
   TileIt ti{b % C.tiles_x, ty0, min(ty0 + C.tpb, C.tiles_y)};
   double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
   if (__builtin_expect((M > 0) && phase == PH_CG, 1)) {
-    if constexpr (M > 0) region_cg_f32<M>(C, A, b, acc, lds, L);
+    if constexpr (M > 0) region_cg_f32<M, XD>(C, A, b, acc, lds, L);
     block_partials_k<3, 0u>(acc, sm, part_out);
   } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
     if (phase == PH_P) {
@@ -1146,14 +1163,23 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   const int lg = g - C.blk0;
   const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
   if constexpr (sizeof(T) == sizeof(float)) {
-    switch (C.tpb) {
-      case 1: round_body<T, U, 1>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-      case 2: round_body<T, U, 2>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-      case 4: round_body<T, U, 4>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-      default: round_body<T, U, 8>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+    if (C.xd) {
+      switch (C.tpb) {
+        case 1: round_body<T, U, 1, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body<T, U, 2, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 4: round_body<T, U, 4, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body<T, U, 8, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      }
+    } else {
+      switch (C.tpb) {
+        case 1: round_body<T, U, 1, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body<T, U, 2, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 4: round_body<T, U, 4, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body<T, U, 8, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      }
     }
   } else {
-    round_body<T, U, 0>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
+    round_body<T, U, 0, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
   }
 }
 

@@ -217,6 +217,19 @@ class DeviceBatch:
         H.check(self.L.pf_batch_poll(self.handle, C.byref(d)), "pf_batch_poll")
         return bool(d.value)
 
+    # Continuous batching: arm one cavity while the others keep running,
+    # queue a chunk of rounds, and collect the cavities that completed.
+    def start(self, slot: int, nsteps: int = 1) -> None:
+        H.check(self.L.pf_batch_start(self.handle, int(slot), int(nsteps)), "pf_batch_start")
+
+    def advance(self, rounds: int) -> None:
+        H.check(self.L.pf_batch_advance(self.handle, int(rounds)), "pf_batch_advance")
+
+    def finished(self) -> list[int]:
+        f = np.zeros(self.max_cav, dtype=np.int32)
+        H.check(self.L.pf_batch_finished(self.handle, _ptr(f)), "pf_batch_finished")
+        return [int(s) for s in np.nonzero(f)[0] if int(s) in self.slots]
+
     def diag(self) -> dict[int, H.StepDiag]:
         diags = (H.StepDiag * self.max_cav)()
         H.check(self.L.pf_batch_diag(self.handle, C.cast(diags, C.c_void_p)), "pf_batch_diag")

@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "pf_batch_zero_fields", "pf_batch_set_meta", "pf_batch_set_rhs", "pf_batch_get_solution",
     "pf_batch_set_dt", "pf_batch_run", "pf_batch_launch", "pf_batch_poll", "pf_batch_diag",
     "pf_batch_centerline", "pf_batch_rounds", "pf_batch_set_timing", "pf_batch_timing", "pf_op_apply",
+    "pf_batch_start", "pf_batch_advance", "pf_batch_finished",
 )
 
 
@@ -115,6 +116,9 @@ def _declare(L):
         "pf_batch_set_timing": ([P, I], I),
         "pf_batch_timing": ([P, C.POINTER(D), C.POINTER(I64)], I),
         "pf_op_apply": ([I, I, I, D, D, P, P, P, P, P, P], I),
+        "pf_batch_start": ([P, I, I], I),
+        "pf_batch_advance": ([P, I64], I),
+        "pf_batch_finished": ([P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

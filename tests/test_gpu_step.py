@@ -346,3 +346,75 @@ def test_batch_alone_equals_in_mix():
         assert all(d.steps_done == 6 for d in dm.values())
         alone.close()
         mix.close()
+
+
+def test_continuous_batching_matches_standalone():
+    """Cavities admitted into a running batch, advanced in chunks next to
+    others at different points of their steps, retired when done and re-armed
+    for more steps give the same bits as each cavity run alone."""
+    from pyflow._device import DeviceBatch, cavity_params
+    mk = lambda n, re: cavity_params(n, n, dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=re,  # noqa: E731
+                                     lin_tol=1e-6, lin_maxiter=3000)
+    pa, pb = mk(40, 400.0), mk(24, 100.0)
+    for prec in ("fp64", "mixed"):
+        ref_a = DeviceBatch([pa], prec)
+        ref_a.run(7)
+        ua_ref = np.zeros((42, 42)); ref_a.download(0, ua_ref, None, None)
+        ref_b = DeviceBatch([pb], prec)
+        db_ref = ref_b.run(3)[0]
+        ub_ref = np.zeros((26, 26)); ref_b.download(0, ub_ref, None, None)
+        srv = DeviceBatch(None, prec, max_cav=4, max_tiles=64)
+        sa = srv.add(pa)
+        srv.start(sa, 5)
+        done, sb, chunks, retired_b = set(), None, 0, None
+        while chunks < 100000:
+            srv.advance(7)  # an odd chunk length: admissions land on both round parities
+            while not srv.poll():
+                pass
+            chunks += 1
+            if chunks == 3:
+                sb = srv.add(pb)
+                srv.start(sb, 3)
+            for s in srv.finished():
+                if s == sa and "a5" not in done:
+                    done.add("a5")
+                    srv.start(sa, 2)  # continue the same cavity for two more steps
+                elif s == sa and "a5" in done:
+                    done.add("a7")
+                elif s == sb and retired_b is None:
+                    retired_b = srv.diag()[sb]
+                    ub = np.zeros((26, 26)); srv.download(sb, ub, None, None)
+                    srv.remove(sb)
+                    done.add("b")
+            if "a7" in done and "b" in done:
+                break
+        assert "a7" in done and "b" in done, (prec, chunks)
+        ua = np.zeros((42, 42)); srv.download(sa, ua, None, None)
+        assert np.array_equal(ua, ua_ref), prec
+        assert np.array_equal(ub, ub_ref), prec
+        assert retired_b.rp_iterations == db_ref.rp_iterations and retired_b.continuity == db_ref.continuity
+        for b in (ref_a, ref_b, srv):
+            b.close()
+
+
+def test_uneven_batch_equals_alone_mixed():
+    """Regression: eight cavities of uneven sizes (partial tile columns, one to
+    49 regions each) advanced together in mixed precision give the same bits
+    as each alone. Streaming stores in the PCG round once broke this."""
+    from pyflow._device import DeviceBatch, cavity_params
+    sizes = [448, 352, 288, 160, 160, 32, 64, 32]
+    res = [1200.0, 300.0, 2500.0, 150.0, 900.0, 4000.0, 100.0, 700.0]
+    mk = lambda n, re: cavity_params(n, n, dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=re,  # noqa: E731
+                                     lin_tol=1e-6, lin_maxiter=10000)
+    ps = [mk(n, re) for n, re in zip(sizes, res)]
+    mix = DeviceBatch(ps, "mixed")
+    dm = mix.run(5)
+    for k, (n, p) in enumerate(zip(sizes, ps)):
+        alone = DeviceBatch([p], "mixed")
+        da = alone.run(5)[0]
+        ua = np.zeros((n + 2, n + 2)); alone.download(0, ua, None, None)
+        um = np.zeros((n + 2, n + 2)); mix.download(k, um, None, None)
+        alone.close()
+        assert np.array_equal(ua, um), (k, n)
+        assert da.rp_iterations_run == dm[k].rp_iterations_run, (k, n)
+    mix.close()
