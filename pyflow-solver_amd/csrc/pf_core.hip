@@ -80,6 +80,7 @@ struct pf_batch {
   int total_blocks = 0;
   int single_slot = -1;  // the only used slot when the batch holds one cavity, else -1
   int max_tpb = 1;       // largest region height (tiles) over the batch's cavities
+  bool any_xd = false;   // some cavity keeps its PCG iterate in FP64
   bool tables_dirty = true;
   int64_t rounds = 0;
   int64_t predicted_rounds = 0;
@@ -201,8 +202,12 @@ static int sync_tables(pf_batch* b) {
     }
   b->single_slot = (nused == 1) ? last : -1;
   b->max_tpb = 1;
+  b->any_xd = false;
   for (int s = 0; s < b->max_cav; ++s)
-    if (b->used[s]) b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
+    if (b->used[s]) {
+      b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
+      b->any_xd = b->any_xd || b->hcav[s].xd;
+    }
   HIPCHK(hipMemcpyAsync(b->dcav, b->hcav.data(), sizeof(PfCav) * b->max_cav, hipMemcpyHostToDevice, b->stream));
   if (!tiles.empty())
     HIPCHK(hipMemcpyAsync(b->dtile, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, b->stream));
@@ -247,10 +252,18 @@ template <class T, int U>
 static void launch_round_u(pf_batch* b, const KArgs& K) {
   dim3 blk(TX, TY);
   const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)region_lds_floats(b->max_tpb) * sizeof(float) : 0;
-  if (b->single_slot >= 0)
-    k_round<T, U, true><<<dim3(b->total_blocks), blk, lds, b->stream>>>(K);
-  else
-    k_round<T, U, false><<<dim3(b->total_blocks), blk, lds, b->stream>>>(K);
+  const dim3 grid(b->total_blocks);
+  if (sizeof(T) == sizeof(float) && b->any_xd) {
+    // FP64-iterate cavities are large (a round takes 100+ us): the descriptor
+    // is read from the table (the by-value copy would only add register
+    // pressure to the bigger kernel).
+    k_round<T, U, false, true><<<grid, blk, lds, b->stream>>>(K);
+  } else {
+    if (b->single_slot >= 0)
+      k_round<T, U, true, false><<<grid, blk, lds, b->stream>>>(K);
+    else
+      k_round<T, U, false, false><<<grid, blk, lds, b->stream>>>(K);
+  }
 }
 
 template <class T>
@@ -378,9 +391,12 @@ int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* pa
 // LDS, above the 64 KB a launch may request without opting in.
 static int allow_region_lds() {
   const int bytes = region_lds_floats(8) * (int)sizeof(float);
-  const void* fns[6] = {(const void*)k_round<float, 1, true>, (const void*)k_round<float, 2, true>,
-                        (const void*)k_round<float, 4, true>, (const void*)k_round<float, 1, false>,
-                        (const void*)k_round<float, 2, false>, (const void*)k_round<float, 4, false>};
+  const void* fns[9] = {
+      (const void*)k_round<float, 1, true, false>,  (const void*)k_round<float, 2, true, false>,
+      (const void*)k_round<float, 4, true, false>,  (const void*)k_round<float, 1, false, false>,
+      (const void*)k_round<float, 2, false, false>, (const void*)k_round<float, 4, false, false>,
+      (const void*)k_round<float, 1, false, true>,  (const void*)k_round<float, 2, false, true>,
+      (const void*)k_round<float, 4, false, true>};
   for (const void* f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   return PF_OK;
 }

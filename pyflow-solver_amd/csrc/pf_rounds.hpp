@@ -31,8 +31,9 @@ constexpr int TX = 64;        // tile width = wavefront width
 constexpr int TY = 16;        // tile height; one wave per tile row
 constexpr int NTH = TX * TY;  // threads per workgroup
 constexpr int NWAVE = NTH / 64;
-constexpr int NP = 4;         // FP64 partial slots per workgroup (stride)
-constexpr int KMAX = 3;       // slots any phase uses
+constexpr int NP = 8;         // FP64 partial slots per workgroup (stride)
+constexpr int KMAX = 3;       // slots of every phase except the FP64-iterate PCG round
+constexpr int KXD = 6;        // slots of the FP64-iterate PCG round
 
 enum Phase {
   PH_IDLE = 0,
@@ -209,9 +210,9 @@ __device__ inline void wave_reduce_phase(int ph, double (&v)[NP]) {
 // ---------------------------------------------------------------------------
 // First phase of a refinement pass: with an FP64 iterate there is no FP32
 // correction to fold in, so the pass starts with the FP64 residual.
-__device__ inline int outer_entry(const PfCav& C) { return C.xd ? PH_OUTER_B : PH_OUTER_A; }
+__device__ __attribute__((always_inline)) inline int outer_entry(const PfCav& C) { return C.xd ? PH_OUTER_B : PH_OUTER_A; }
 
-__device__ inline void cg_next(const PfCav& C, PfScal& S) {
+__device__ __attribute__((always_inline)) inline void cg_next(const PfCav& C, PfScal& S) {
   const bool mixed = (C.prec == PF_PREC_MIXED);
   if (S.it_total >= C.maxiter) {
     S.info = C.maxiter;
@@ -233,7 +234,7 @@ __device__ inline void enter_done(PfScal& S) {
 
 // Tail of a MIXED PCG iteration: beta = gamma'/gamma, alpha' = gamma'/(delta' -
 // beta gamma'/alpha). Pending phase PH_CG is the common case of a round.
-__device__ inline void tail_cg(const PfCav& C, PfScal& S, const double* tot) {
+__device__ __attribute__((always_inline)) inline void tail_cg(const PfCav& C, PfScal& S, const double* tot) {
   S.pending = PH_IDLE;
   const double g1 = tot[0], d1 = tot[1];
   const double beta = g1 / S.rho;
@@ -247,7 +248,30 @@ __device__ inline void tail_cg(const PfCav& C, PfScal& S, const double* tot) {
   cg_next(C, S);
 }
 
-__attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
+__attribute__((noinline)) // Tail of an FP64-iterate PCG iteration. Besides gamma' = r'.u', delta' =
+// (A u').u' and |r'|^2 the round returned pi = p.(A p), sigma = u'.(A p) and
+// rho = r'.p of the direction p just used, so p' = u' + beta p has
+//   p'.(A p') = delta' + 2 beta sigma + beta^2 pi   (A is symmetric on the
+//   vectors the solve produces: their pinned cell stays 0)
+// and the step along p' is the exact line minimum alpha' = r'.p' / p'.(A p')
+// with r'.p' = gamma' + beta rho. No orthogonality between successive
+// residuals is assumed, which the FP32 vectors lose on large grids.
+__device__ __attribute__((always_inline)) inline void tail_cg_xd(const PfCav& C, PfScal& S, const double* tot) {
+  S.pending = PH_IDLE;
+  const double g1 = tot[0], d1 = tot[1], pi = tot[3], sig = tot[4], rho = tot[5];
+  const double beta = g1 / S.rho;
+  const double pap = d1 + 2.0 * beta * sig + beta * beta * pi;
+  S.alpha = (g1 + beta * rho) / pap;
+  S.beta = beta;
+  S.rho = g1;
+  S.rnorm2 = tot[2];
+  S.it += 1;
+  S.it_total += 1;
+  S.it_run += 1;
+  cg_next(C, S);
+}
+
+__device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
   const int ph = S.pending;
   S.pending = PH_IDLE;
   switch (ph) {
@@ -530,7 +554,7 @@ __device__ inline void cell_q(const PfCav& C, const PhaseArgs& A, int j, int i, 
 
 // Float32 helpers of the mixed-precision P/Q: Jacobi weight and
 // d = fma(d_old, beta, z) in float32.
-__device__ inline float minv_f32(const PfCav& C, const PhaseArgs& A, int j, int i) {
+__device__ __attribute__((always_inline)) inline float minv_f32(const PfCav& C, const PhaseArgs& A, int j, int i) {
   if (!C.jacobi) return 1.0f;
   const bool bnd = (i == 0 || i == C.nx - 1 || j == 0 || j == C.ny - 1);
   return bnd ? 1.0f : A.minvf;
@@ -793,7 +817,7 @@ struct RegionIn {
 };
 
 template <int M, bool XD>
-__device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M, XD>& L) {
+__device__ __attribute__((always_inline)) inline void region_load(const PfCav& C, int par, int b, RegionIn<M, XD>& L) {
   const int nx = C.nx, ny = C.ny;
   const float* __restrict__ R = mixed_r(C, par);
   const float* __restrict__ Pold = mixed_p(C, par ^ 1);
@@ -850,7 +874,7 @@ __device__ inline void region_load(const PfCav& C, int par, int b, RegionIn<M, X
 }
 
 template <int M, bool XD>
-__device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, double* acc,
+__device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, double* acc,
                                      float* __restrict__ lds, const RegionIn<M, XD>& L) {
   const int nx = C.nx, ny = C.ny;
   float* __restrict__ Rn = mixed_r(C, A.par ^ 1);
@@ -869,6 +893,7 @@ __device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, 
   const int i = i0 + li;
   // Pass 2.
   bool in[M];
+  float pkr[M];  // p_k at the cells (FP64-iterate partials)
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = j0 + TY * m + lj;
@@ -887,6 +912,7 @@ __device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, 
         X[k] = L.xo[m] + al * pk;
     }
     lp[(TY * m + lj + 2) * HP + li + 2] = pk;
+    pkr[m] = pk;
   }
   if (L.hrow >= 0) {
     const int hj = j0 + L.hrow - 2, hi = i0 + L.hcol - 2;
@@ -915,6 +941,11 @@ __device__ inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, 
       const float sv = s_at(row, li + 2, j, i);
       rk1[m] = L.ro[m] - al * sv;
       uk1[m] = minv_f32(C, A, j, i) * rk1[m];
+      if constexpr (XD) {
+        acc[3] += (double)pkr[m] * (double)sv;
+        acc[4] += (double)uk1[m] * (double)sv;
+        acc[5] += (double)rk1[m] * (double)pkr[m];
+      }
       Rn[(size_t)j * nx + i] = rk1[m];
     }
     lu[row * HP + li + 2] = uk1[m];
@@ -1013,7 +1044,7 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
 // pays one memory round trip before its data is in registers. They are wasted
 // in the few rounds that run another phase.
 template <class T, int U, int M, bool XD>
-__device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
+__device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
                                   double (*sm)[NP], double* tot, float* lds) {
   const int parity = KA.parity;
   const int tid = tid_flat();
@@ -1029,15 +1060,19 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
   const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
   const int nblk = C.nblk;
   const int lane = tid & 63;
-  const bool w0 = tid < 64;
+  // Folding waves: wave 0 takes slots 0..2; in the FP64-iterate kernel wave 1
+  // takes slots 3..5 (one wave holding all six would need 48 more VGPRs).
+  const int fwave = tid >> 6;
+  const bool wf = XD ? (fwave < 2) : (fwave == 0);
+  const int koff = (XD && fwave == 1) ? KMAX : 0;
   constexpr int PRE = 4;
   double pre[PRE][KMAX];
 #pragma unroll
   for (int t = 0; t < PRE; ++t) {
     const int q = lane + 64 * t;
-    const size_t qq = (w0 && q < nblk) ? (size_t)q : 0;
+    const size_t qq = (wf && q < nblk) ? (size_t)q : 0;
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[qq * NP + k];
+    for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[qq * NP + koff + k];
   }
   const uint32_t sw = src[tid < NW32 ? tid : 0];
   RegionIn<(M > 0 ? M : 1), XD> L;
@@ -1045,14 +1080,14 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
   if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
   __syncthreads();
   if (st.pending != PH_IDLE) {
-    // Wave 0 folds the partials in a fixed order (per lane: increasing
-    // workgroup index; then the DPP tree), so every workgroup of the cavity
-    // computes the same bits; lane 63 runs the tail.
-    if (w0) {
-      // A PCG iteration is pending in nearly every round: its fold (sums
-      // only), reduction and tail form one straight-line block.
-      const bool hot = (M > 0) && st.pending == PH_CG;
-      const unsigned mm = hot ? 0u : phase_maxmask(st.pending);
+    // The folding waves sum the partials in a fixed order (per lane:
+    // increasing workgroup index; then the DPP tree), so every workgroup of
+    // the cavity computes the same bits; thread 63 runs the tail. A PCG
+    // iteration is pending in nearly every round: its fold (sums only),
+    // reduction and tail form one straight-line block.
+    const bool hot = (M > 0) && st.pending == PH_CG;
+    if (wf) {
+      const unsigned mm = hot ? 0u : (phase_maxmask(st.pending) >> koff);
       double pa[NP] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int t = 0; t < PRE; ++t)
@@ -1062,22 +1097,26 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
         }
       for (int q = lane + 64 * PRE; q < nblk; q += 64) {
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) pa[k] = redop(mm, k, pa[k], prev[(size_t)q * NP + k]);
+        for (int k = 0; k < KMAX; ++k) pa[k] = redop(mm, k, pa[k], prev[(size_t)q * NP + koff + k]);
       }
-      if (__builtin_expect(hot, 1)) {
-        wave_reduce_k<3, 0u>(pa);
-        if (lane == 63) {
-#pragma unroll
-          for (int k = 0; k < NP; ++k) tot[k] = pa[k];
-          tail_cg(C, st, tot);
-        }
-      } else {
+      if (__builtin_expect(hot, 1))
+        wave_reduce_k<KMAX, 0u>(pa);
+      else
         wave_reduce_phase(st.pending, pa);
-        if (lane == 63) {
+      if (lane == 63) {
 #pragma unroll
-          for (int k = 0; k < NP; ++k) tot[k] = pa[k];
-          tail(*Cg, st, tot);
-        }
+        for (int k = 0; k < KMAX; ++k) tot[koff + k] = pa[k];
+      }
+    }
+    if constexpr (XD) __syncthreads();  // wave 1's slots visible to thread 63
+    if (tid == 63) {
+      if (__builtin_expect(hot, 1)) {
+        if constexpr (XD)
+          tail_cg_xd(C, st, tot);
+        else
+          tail_cg(C, st, tot);
+      } else {
+        tail(*Cg, st, tot);
       }
     }
     __syncthreads();  // tail result visible
@@ -1115,7 +1154,7 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
   double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
   if (__builtin_expect((M > 0) && phase == PH_CG, 1)) {
     if constexpr (M > 0) region_cg_f32<M, XD>(C, A, b, acc, lds, L);
-    block_partials_k<3, 0u>(acc, sm, part_out);
+    block_partials_k<(XD ? KXD : 3), 0u>(acc, sm, part_out);
   } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
     if (phase == PH_P) {
       for (int m = 0; m < C.tpb; m += U) chunk_p<T, U>(C, A, ti, acc);
@@ -1135,7 +1174,9 @@ __device__ inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* 
   }
 }
 
-template <class T, int U, bool SINGLE>
+// XDANY: some cavity of the batch keeps its PCG iterate in FP64; otherwise
+// only the FP32-iterate body is compiled, which keeps the kernel small.
+template <class T, int U, bool SINGLE, bool XDANY>
 __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   __shared__ PfScal st;
   __shared__ double sm[NWAVE][NP];
@@ -1163,7 +1204,7 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   const int lg = g - C.blk0;
   const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
   if constexpr (sizeof(T) == sizeof(float)) {
-    if (C.xd) {
+    if (XDANY && C.xd) {
       switch (C.tpb) {
         case 1: round_body<T, U, 1, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
         case 2: round_body<T, U, 2, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;

@@ -130,3 +130,18 @@ def test_device_cg_mixed_reaches_fp64_tolerance(n):
         assert rnorm <= tol * np.linalg.norm(b)
         mv = lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0)  # noqa: E731
         assert float(np.linalg.norm(b - mv(x))) <= tol * np.linalg.norm(b) * 1.0001
+
+
+@pytest.mark.parametrize("n", [64, 200])
+def test_device_cg_mixed_fp64_iterate(n, monkeypatch):
+    """The FP64-iterate PCG (exact line search along each direction, used on
+    large grids) meets the FP64 stopping rule on the true residual."""
+    from pyflow.linear_solvers.interface import PinnedNegLaplacian, device_cg
+    monkeypatch.setenv("PYFLOW_XD", "1")
+    b = _rhs(n, n, 11)
+    op = PinnedNegLaplacian(n, n, 1.0, 1.0)
+    mv = lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0)  # noqa: E731
+    for tol in (1e-6, 1e-10):
+        x, info, its, rnorm, conv = device_cg(op, b, tol, 20000, jacobi=True, precision="mixed")
+        assert info == 0 and conv
+        assert float(np.linalg.norm(b - mv(x))) <= tol * np.linalg.norm(b) * 1.0001
