@@ -786,10 +786,9 @@ __device__ inline void cell_cg0(const PfCav& C, const PhaseArgs& A, int j, int i
 
 // One float32 PCG iteration over a workgroup's region (64 columns x RH = 16*M
 // rows), staged through LDS with a 2-cell halo:
-//   pass 1  (region_load, issued at kernel entry together with the state and
-//           partials loads, before the phase is known) r_k, p_{k-1}, x at the
-//           thread's M cells and r_k, p_{k-1} at (at most) one cell of the
-//           2-cell ring;
+//   pass 1  (region_load) every global load of the thread, issued together:
+//           r_k, p_{k-1}, x at its M cells and r_k, p_{k-1} at (at most) one
+//           cell of the 2-cell ring;
 //   pass 2  p_k into LDS (cells and ring), r_k of the ring into LDS; store
 //           p_k and x_{k+1};
 //   pass 3  s_k = A p_k, r_{k+1}, u_{k+1} at the cells and the 1-cell ring;
@@ -828,9 +827,8 @@ __device__ __attribute__((always_inline)) inline void region_load(const PfCav& C
   const int i0 = (b % C.tiles_x) * TX;
   const int j0 = (b / C.tiles_x) * RH;
   const int i = i0 + li;
-  // Loads are unconditional (out-of-range lanes read cell 0 and discard it):
-  // loads under a branch make the compiler wait for all of them at the join,
-  // before the state has even been stored to LDS.
+  // Loads are unconditional (out-of-range lanes read cell 0 and discard it),
+  // so none of them sits behind a branch and all are in flight together.
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = j0 + TY * m + lj;
@@ -1039,10 +1037,9 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
 }
 
 // Body of one round for workgroup b of cavity c. M > 0 (float32 only): the
-// cavity's region height in tiles; the region loads of a PCG iteration are
-// issued at entry, together with the state and partials loads, so the round
-// pays one memory round trip before its data is in registers. They are wasted
-// in the few rounds that run another phase.
+// cavity's region height in tiles. (Issuing the PCG region loads at entry,
+// before the phase is known, gained no time and gave wrong results in
+// batches with more workgroups than CUs; they are issued in the PCG branch.)
 template <class T, int U, int M, bool XD>
 __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
                                   double (*sm)[NP], double* tot, float* lds) {
@@ -1052,11 +1049,10 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   uint32_t* dst = (uint32_t*)(KA.scal + (size_t)parity * KA.max_cav + c);
   constexpr int NW32 = (int)(sizeof(PfScal) / 4);
   static_assert(NW32 <= NTH, "state copied with one word per thread");
-  // The state words and the previous round's partials (wave 0 only: lane l
-  // takes workgroups l, l+64, l+128, l+192) are loaded together, before
+  // The state words and the previous round's partials (folding waves only:
+  // lane l takes workgroups l, l+64, l+128, l+192) are loaded together, before
   // anything is known about the phase. Loads are unconditional (out-of-range
-  // lanes read slot 0 and discard it), so the compiler can count them and let
-  // the region loads below stay in flight.
+  // lanes read slot 0 and discard it).
   const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
   const int nblk = C.nblk;
   const int lane = tid & 63;
@@ -1075,8 +1071,6 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
     for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[qq * NP + koff + k];
   }
   const uint32_t sw = src[tid < NW32 ? tid : 0];
-  RegionIn<(M > 0 ? M : 1), XD> L;
-  if constexpr (M > 0) region_load<M, XD>(C, parity, b, L);
   if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
   __syncthreads();
   if (st.pending != PH_IDLE) {
@@ -1153,7 +1147,11 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   TileIt ti{b % C.tiles_x, ty0, min(ty0 + C.tpb, C.tiles_y)};
   double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
   if (__builtin_expect((M > 0) && phase == PH_CG, 1)) {
-    if constexpr (M > 0) region_cg_f32<M, XD>(C, A, b, acc, lds, L);
+    if constexpr (M > 0) {
+      RegionIn<M, XD> L;
+      region_load<M, XD>(C, parity, b, L);
+      region_cg_f32<M, XD>(C, A, b, acc, lds, L);
+    }
     block_partials_k<(XD ? KXD : 3), 0u>(acc, sm, part_out);
   } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
     if (phase == PH_P) {

@@ -418,3 +418,28 @@ def test_uneven_batch_equals_alone_mixed():
         assert np.array_equal(ua, um), (k, n)
         assert da.rp_iterations_run == dm[k].rp_iterations_run, (k, n)
     mix.close()
+
+
+def test_batch_more_workgroups_than_cus_mixed():
+    """Regression: a mixed-precision batch whose rounds hold more workgroups
+    than the GPU has CUs (sixteen cavities of 32..512 cells a side, 400+
+    regions), so some workgroups of a round run after others have finished,
+    gives each cavity the same bits as alone."""
+    from pyflow._device import DeviceBatch, cavity_params
+    sizes = [448, 352, 288, 160, 160, 32, 64, 32, 96, 448, 352, 480, 288, 320, 512, 384]
+    mk = lambda n, k: cavity_params(n, n, dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0,  # noqa: E731
+                                    Re=150.0 + 97.0 * k, lin_tol=1e-6, lin_maxiter=10000)
+    ps = [mk(n, k) for k, n in enumerate(sizes)]
+    mix = DeviceBatch(ps, "mixed")
+    dm = mix.run(2)
+    assert not any(d.nan_detected for d in dm.values())
+    for k in (0, 3, 8, 11, 14):
+        n = sizes[k]
+        alone = DeviceBatch([ps[k]], "mixed")
+        da = alone.run(2)[0]
+        ua = np.zeros((n + 2, n + 2)); alone.download(0, ua, None, None)
+        um = np.zeros((n + 2, n + 2)); mix.download(k, um, None, None)
+        alone.close()
+        assert np.array_equal(ua, um), (k, n)
+        assert da.rp_iterations_run == dm[k].rp_iterations_run, (k, n)
+    mix.close()
