@@ -3,7 +3,8 @@
 // Execution model
 // ---------------
 // A batch holds cavities (independent lid-driven-cavity problems of any size).
-// Each cavity is cut into 64 x 16-cell tiles and covered by nblk workgroups of
+// Each cavity is cut into 64 x 16-cell tiles, grouped into regions of tpb tiles
+// stacked in y, and covered by nblk workgroups (rpw regions each) of
 // 64 x 16 threads (one wave per tile row), each owning tpb consecutive tiles.
 // Every cavity runs a phase machine on the device:
 //
@@ -81,6 +82,7 @@ struct pf_batch {
   int single_slot = -1;  // the only used slot when the batch holds one cavity, else -1
   int max_tpb = 1;       // largest region height (tiles) over the batch's cavities
   bool any_xd = false;   // some cavity keeps its PCG iterate in FP64
+  bool multi = false;    // some cavity has several regions per workgroup
   bool tables_dirty = true;
   int64_t rounds = 0;
   int64_t predicted_rounds = 0;
@@ -124,22 +126,31 @@ static int env_int(const char* name, int dflt) {
 // Workgroup regions: M = tpb tiles (64 x 16 cells each) stacked in y, M a
 // power of two <= 8, starting from PYFLOW_TPB (default 4) and doubled while a
 // cavity would need more than PYFLOW_MAXBLK (default 1024) workgroups.
-static void blocking_of(int tiles_x, int tiles_y, int* nblk, int* tpb) {
+// Regions of 64 x 16*tpb cells (tpb = 4 unless the cavity is shorter), and
+// rpw consecutive regions per workgroup so that a cavity launches at most
+// PYFLOW_MAXWG (256: one per CU) workgroups: a workgroup then pays its
+// state/partials round trip and tail once per round, not once per region.
+static void blocking_of(int tiles_x, int tiles_y, int* nblk, int* tpb, int* nreg, int* rpw) {
   int m = 1;
   const int target = env_int("PYFLOW_TPB", 4);
   while (m * 2 <= target && m < 8) m *= 2;
   while (m > 1 && m / 2 >= tiles_y) m /= 2;  // no more rows than the cavity has
-  const int maxblk = env_int("PYFLOW_MAXBLK", 1024);
-  while (m < 8 && (int64_t)tiles_x * ((tiles_y + m - 1) / m) > maxblk) m *= 2;
+  const int regions = tiles_x * ((tiles_y + m - 1) / m);
+  const int maxwg = std::max(1, env_int("PYFLOW_MAXWG", 256));
+  int k = 1;
+  while ((regions + k - 1) / k > maxwg) k *= 2;
   *tpb = m;
-  *nblk = tiles_x * ((tiles_y + m - 1) / m);
+  *nreg = regions;
+  *rpw = k;
+  *nblk = (regions + k - 1) / k;
 }
 
 static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   const size_t n = (size_t)p->nx * p->ny;
   int tx, ty, nb, tpb;
   tiles_of(p->nx, p->ny, &tx, &ty);
-  blocking_of(tx, ty, &nb, &tpb);
+  int nreg, rpw;
+  blocking_of(tx, ty, &nb, &tpb, &nreg, &rpw);
   size_t s = 0;
   s += 10 * align256(n * esize(prec));  // u v p us vs r d0 d1 x r1
   s += align256(n * sizeof(double));                           // b
@@ -155,7 +166,7 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   c.ntiles = tiles_of(p->nx, p->ny, &tx, &ty);
   c.tiles_x = tx;
   c.tiles_y = ty;
-  blocking_of(tx, ty, &c.nblk, &c.tpb);
+  blocking_of(tx, ty, &c.nblk, &c.tpb, &c.nreg, &c.rpw);
   c.scheme = p->scheme;
   c.lid_on = p->lid_enabled ? 1 : 0;
   c.jacobi = p->jacobi ? 1 : 0;
@@ -203,10 +214,12 @@ static int sync_tables(pf_batch* b) {
   b->single_slot = (nused == 1) ? last : -1;
   b->max_tpb = 1;
   b->any_xd = false;
+  b->multi = false;
   for (int s = 0; s < b->max_cav; ++s)
     if (b->used[s]) {
       b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
       b->any_xd = b->any_xd || b->hcav[s].xd;
+      b->multi = b->multi || b->hcav[s].rpw > 1;
     }
   HIPCHK(hipMemcpyAsync(b->dcav, b->hcav.data(), sizeof(PfCav) * b->max_cav, hipMemcpyHostToDevice, b->stream));
   if (!tiles.empty())
@@ -248,8 +261,8 @@ static int check_slot(pf_batch* b, int slot) {
   return PF_OK;
 }
 
-template <class T, int U>
-static void launch_round_u(pf_batch* b, const KArgs& K) {
+template <class T, int U, bool MULTI>
+static void launch_round_m(pf_batch* b, const KArgs& K) {
   dim3 blk(TX, TY);
   const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)region_lds_floats(b->max_tpb) * sizeof(float) : 0;
   const dim3 grid(b->total_blocks);
@@ -257,13 +270,21 @@ static void launch_round_u(pf_batch* b, const KArgs& K) {
     // FP64-iterate cavities are large (a round takes 100+ us): the descriptor
     // is read from the table (the by-value copy would only add register
     // pressure to the bigger kernel).
-    k_round<T, U, false, true><<<grid, blk, lds, b->stream>>>(K);
+    k_round<T, U, false, true, MULTI><<<grid, blk, lds, b->stream>>>(K);
   } else {
     if (b->single_slot >= 0)
-      k_round<T, U, true, false><<<grid, blk, lds, b->stream>>>(K);
+      k_round<T, U, true, false, MULTI><<<grid, blk, lds, b->stream>>>(K);
     else
-      k_round<T, U, false, false><<<grid, blk, lds, b->stream>>>(K);
+      k_round<T, U, false, false, MULTI><<<grid, blk, lds, b->stream>>>(K);
   }
+}
+
+template <class T, int U>
+static void launch_round_u(pf_batch* b, const KArgs& K) {
+  if (b->multi)
+    launch_round_m<T, U, true>(b, K);
+  else
+    launch_round_m<T, U, false>(b, K);
 }
 
 template <class T>
@@ -279,12 +300,16 @@ static void launch_round(pf_batch* b) {
   K.parity = (int)(b->round & 1);
   K.slot0 = b->single_slot;
   if (b->single_slot >= 0) K.cav0 = b->hcav[b->single_slot];
-  if (b->unroll >= 4)
-    launch_round_u<T, 4>(b, K);
-  else if (b->unroll == 2)
-    launch_round_u<T, 2>(b, K);
-  else
-    launch_round_u<T, 1>(b, K);
+  if constexpr (sizeof(T) == sizeof(float)) {
+    launch_round_u<T, 2>(b, K);  // U only shapes the FP64 P/Q loops
+  } else {
+    if (b->unroll >= 4)
+      launch_round_u<T, 4>(b, K);
+    else if (b->unroll == 2)
+      launch_round_u<T, 2>(b, K);
+    else
+      launch_round_u<T, 1>(b, K);
+  }
   b->round += 1;
 }
 
@@ -391,12 +416,10 @@ int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* pa
 // LDS, above the 64 KB a launch may request without opting in.
 static int allow_region_lds() {
   const int bytes = region_lds_floats(8) * (int)sizeof(float);
-  const void* fns[9] = {
-      (const void*)k_round<float, 1, true, false>,  (const void*)k_round<float, 2, true, false>,
-      (const void*)k_round<float, 4, true, false>,  (const void*)k_round<float, 1, false, false>,
-      (const void*)k_round<float, 2, false, false>, (const void*)k_round<float, 4, false, false>,
-      (const void*)k_round<float, 1, false, true>,  (const void*)k_round<float, 2, false, true>,
-      (const void*)k_round<float, 4, false, true>};
+  const void* fns[6] = {
+      (const void*)k_round<float, 2, true, false, false>, (const void*)k_round<float, 2, false, false, false>,
+      (const void*)k_round<float, 2, false, true, false>, (const void*)k_round<float, 2, true, false, true>,
+      (const void*)k_round<float, 2, false, false, true>, (const void*)k_round<float, 2, false, true, true>};
   for (const void* f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   return PF_OK;
 }

@@ -3,16 +3,16 @@
 //
 // Cross-workgroup protocol (no fences, no spin-waits)
 // -------------------------------------------------
-// Cavity c is covered by nblk workgroups; workgroup b owns tiles
-// [b*tpb, (b+1)*tpb) of 64x16 cells (one cell per thread per tile). In round r
-// every workgroup of c
+// Cavity c is cut into nreg regions of tpb 64x16-cell tiles stacked in y and
+// covered by nblk workgroups; workgroup b owns regions [b*rpw, (b+1)*rpw)
+// (one cell per thread per tile). In round r every workgroup of c
 //   1. copies the cavity's scalar state written in round r-1 (buffer r^1),
 //   2. if that state has a pending phase, sums the pending phase's FP64
 //      partials of all nblk workgroups (written in round r-1, parity r^1) in
 //      a fixed order and runs the phase's scalar tail (dt controller, CG
 //      alpha/beta, convergence tests, phase transition) -- every workgroup
 //      computes the same bits, so they all agree on the next phase,
-//   3. executes that phase on its tiles and writes its own partials (parity r),
+//   3. executes that phase on its regions and writes its own partials (parity r),
 //   4. (workgroup 0 only) stores the state, with the executed phase pending,
 //      into buffer r.
 // Data written in round r is only read in round r+1, so kernel boundaries are
@@ -76,7 +76,8 @@ __device__ __host__ inline unsigned phase_maxmask(int ph) {
 
 // Read-mostly description of one cavity (rewritten only between runs).
 struct PfCav {
-  int nx, ny, tiles_x, tiles_y, ntiles, nblk, blk0, tpb;  // tpb = M: tiles per workgroup, stacked in y
+  int nx, ny, tiles_x, tiles_y, ntiles, nblk, blk0, tpb;  // nblk workgroups; tpb = M: tiles per region, stacked in y
+  int nreg, rpw;  // regions; regions per workgroup (workgroup b owns regions b*rpw .. b*rpw+rpw-1)
   int scheme, lid_on, jacobi, maxiter, op, mode, prec;
   int xd;  // MIXED: the PCG iterate is accumulated in FP64 (x64) instead of FP32
   double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
@@ -816,7 +817,7 @@ struct RegionIn {
 };
 
 template <int M, bool XD>
-__device__ __attribute__((always_inline)) inline void region_load(const PfCav& C, int par, int b, RegionIn<M, XD>& L) {
+__device__ __attribute__((always_inline)) inline void region_load(const PfCav& C, int par, int r, RegionIn<M, XD>& L) {
   const int nx = C.nx, ny = C.ny;
   const float* __restrict__ R = mixed_r(C, par);
   const float* __restrict__ Pold = mixed_p(C, par ^ 1);
@@ -824,8 +825,8 @@ __device__ __attribute__((always_inline)) inline void region_load(const PfCav& C
   constexpr int RH = TY * M;
   const int li = threadIdx.x, lj = threadIdx.y;
   const int tid = lj * TX + li;
-  const int i0 = (b % C.tiles_x) * TX;
-  const int j0 = (b / C.tiles_x) * RH;
+  const int i0 = (r % C.tiles_x) * TX;
+  const int j0 = (r / C.tiles_x) * RH;
   const int i = i0 + li;
   // Loads are unconditional (out-of-range lanes read cell 0 and discard it),
   // so none of them sits behind a branch and all are in flight together.
@@ -872,7 +873,7 @@ __device__ __attribute__((always_inline)) inline void region_load(const PfCav& C
 }
 
 template <int M, bool XD>
-__device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int b, double* acc,
+__device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav& C, const PhaseArgs& A, int r, double* acc,
                                      float* __restrict__ lds, const RegionIn<M, XD>& L) {
   const int nx = C.nx, ny = C.ny;
   float* __restrict__ Rn = mixed_r(C, A.par ^ 1);
@@ -886,8 +887,8 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
   float* __restrict__ lu = lds + NR * HP;  // r_k on the ring, then u_{k+1}
   const int li = threadIdx.x, lj = threadIdx.y;
   const int tid = lj * TX + li;
-  const int i0 = (b % C.tiles_x) * TX;
-  const int j0 = (b / C.tiles_x) * RH;
+  const int i0 = (r % C.tiles_x) * TX;
+  const int j0 = (r / C.tiles_x) * RH;
   const int i = i0 + li;
   // Pass 2.
   bool in[M];
@@ -1024,14 +1025,20 @@ struct KArgs {
 // Cg points at the cavity's descriptor in global memory (a kernel-argument
 // copy passed by reference would have to be spilled to scratch first).
 template <class T>
-__attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A,
-                                                     const TileIt ti, double (*sm)[NP], double* part_out) {
+__attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A, int b,
+                                                     double (*sm)[NP], double* part_out) {
   const PfCav& C = *Cg;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  for (int ty = ti.ty; ty < ti.tyend; ++ty) {
-    const int i = ti.tx * TX + threadIdx.x;
-    const int j = ty * TY + threadIdx.y;
-    if (i < C.nx && j < C.ny) cell_phase<T>(phase, C, A, j, i, acc);
+  for (int k = 0; k < C.rpw; ++k) {
+    const int r = b * C.rpw + k;
+    if (r >= C.nreg) break;
+    const int ty0 = (r / C.tiles_x) * C.tpb;
+    const int tyend = min(ty0 + C.tpb, C.tiles_y);
+    const int i = (r % C.tiles_x) * TX + threadIdx.x;
+    for (int ty = ty0; ty < tyend; ++ty) {
+      const int j = ty * TY + threadIdx.y;
+      if (i < C.nx && j < C.ny) cell_phase<T>(phase, C, A, j, i, acc);
+    }
   }
   if (phase_nparts(phase) > 0) block_partials_phase(phase, acc, sm, part_out);
 }
@@ -1040,7 +1047,7 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
 // cavity's region height in tiles. (Issuing the PCG region loads at entry,
 // before the phase is known, gained no time and gave wrong results in
 // batches with more workgroups than CUs; they are issued in the PCG branch.)
-template <class T, int U, int M, bool XD>
+template <class T, int U, int M, bool XD, bool MULTI>
 __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
                                   double (*sm)[NP], double* tot, float* lds) {
   const int parity = KA.parity;
@@ -1143,26 +1150,36 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   A.alphaf = (float)A.alpha;
   A.minvf = (float)C.invd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  const int ty0 = (b / C.tiles_x) * C.tpb;
-  TileIt ti{b % C.tiles_x, ty0, min(ty0 + C.tpb, C.tiles_y)};
   double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
+  // Workgroup b owns regions b*rpw .. b*rpw+rpw-1 (fewer at the cavity's end);
+  // without MULTI every cavity of the batch has rpw = 1.
+  const int r0 = MULTI ? b * C.rpw : b;
+  const int r1 = MULTI ? min(r0 + C.rpw, C.nreg) : b + 1;
   if (__builtin_expect((M > 0) && phase == PH_CG, 1)) {
     if constexpr (M > 0) {
-      RegionIn<M, XD> L;
-      region_load<M, XD>(C, parity, b, L);
-      region_cg_f32<M, XD>(C, A, b, acc, lds, L);
+      for (int r = r0; r < r1; ++r) {
+        if (r > r0) __syncthreads();  // the previous region's LDS reads are done
+        RegionIn<M, XD> L;
+        region_load<M, XD>(C, parity, r, L);
+        region_cg_f32<M, XD>(C, A, r, acc, lds, L);
+      }
     }
     block_partials_k<(XD ? KXD : 3), 0u>(acc, sm, part_out);
   } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
-    if (phase == PH_P) {
-      for (int m = 0; m < C.tpb; m += U) chunk_p<T, U>(C, A, ti, acc);
-      block_partials_k<1, 0u>(acc, sm, part_out);
-    } else {
-      for (int m = 0; m < C.tpb; m += U) chunk_q<T, U>(C, A, ti, acc);
-      block_partials_k<2, 0u>(acc, sm, part_out);
+    for (int r = r0; r < r1; ++r) {
+      const int ty0 = (r / C.tiles_x) * C.tpb;
+      TileIt ti{r % C.tiles_x, ty0, min(ty0 + C.tpb, C.tiles_y)};
+      if (phase == PH_P)
+        for (int m = 0; m < C.tpb; m += U) chunk_p<T, U>(C, A, ti, acc);
+      else
+        for (int m = 0; m < C.tpb; m += U) chunk_q<T, U>(C, A, ti, acc);
     }
+    if (phase == PH_P)
+      block_partials_k<1, 0u>(acc, sm, part_out);
+    else
+      block_partials_k<2, 0u>(acc, sm, part_out);
   } else {
-    round_cold<T>(phase, Cg, A, ti, sm, part_out);
+    round_cold<T>(phase, Cg, A, b, sm, part_out);
   }
   if (b == 0) {
     __syncthreads();
@@ -1174,7 +1191,8 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
 
 // XDANY: some cavity of the batch keeps its PCG iterate in FP64; otherwise
 // only the FP32-iterate body is compiled, which keeps the kernel small.
-template <class T, int U, bool SINGLE, bool XDANY>
+// MULTI: some cavity has several regions per workgroup.
+template <class T, int U, bool SINGLE, bool XDANY, bool MULTI>
 __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   __shared__ PfScal st;
   __shared__ double sm[NWAVE][NP];
@@ -1204,21 +1222,21 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   if constexpr (sizeof(T) == sizeof(float)) {
     if (XDANY && C.xd) {
       switch (C.tpb) {
-        case 1: round_body<T, U, 1, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        case 2: round_body<T, U, 2, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        case 4: round_body<T, U, 4, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        default: round_body<T, U, 8, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 1: round_body<T, U, 1, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body<T, U, 2, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 4: round_body<T, U, 4, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body<T, U, 8, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
       }
     } else {
       switch (C.tpb) {
-        case 1: round_body<T, U, 1, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        case 2: round_body<T, U, 2, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        case 4: round_body<T, U, 4, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        default: round_body<T, U, 8, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 1: round_body<T, U, 1, false, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body<T, U, 2, false, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 4: round_body<T, U, 4, false, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body<T, U, 8, false, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
       }
     }
   } else {
-    round_body<T, U, 0, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
+    round_body<T, U, 0, false, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
   }
 }
 

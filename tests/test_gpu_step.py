@@ -443,3 +443,27 @@ def test_batch_more_workgroups_than_cus_mixed():
         assert np.array_equal(ua, um), (k, n)
         assert da.rp_iterations_run == dm[k].rp_iterations_run, (k, n)
     mix.close()
+
+
+@pytest.mark.parametrize("precision,cap", [("fp64", 1e-8), ("mixed", 1e-5)])
+def test_several_regions_per_workgroup(precision, cap, monkeypatch):
+    """Workgroups that own several regions (a cavity capped at two workgroups:
+    130x97 cells = 6 regions, 4 + 2 per workgroup) follow the FP64 oracle."""
+    from pyflow._device import DeviceBatch, cavity_params
+    monkeypatch.setenv("PYFLOW_MAXWG", "2")
+    nx, ny = 130, 97
+    p = cavity_params(nx, ny, dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=400.0, lin_tol=1e-10,
+                      lin_maxiter=5000)
+    b = DeviceBatch([p], precision)
+    d = b.run(6)[0]
+    u = np.zeros((ny + 2, nx + 2)); v = np.zeros_like(u)
+    b.download(0, u, v, None)
+    b.close()
+    cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05,
+                          Re=400.0, lid_velocity=1.0, lin_tol=1e-10, lin_maxiter=5000)
+    ref = O.allocate_state(nx, ny)
+    for it in range(6):
+        O.step(cfg, ref, it)
+    assert d.steps_done == 6 and not d.nan_detected
+    assert float(np.max(np.abs(u - ref.fields["u"]))) <= cap
+    assert float(np.max(np.abs(v - ref.fields["v"]))) <= cap
