@@ -529,7 +529,16 @@ def batch_params(count, seed, inner_rtol):
     return params, ns
 
 
+# The 256-cavity batch: at most 16 workgroups per cavity (several regions per
+# workgroup on the larger cavities) -- a round then holds ~1500 instead of ~6000
+# workgroups, each paying its state round trip and tail once (+10% measured).
+# Served requests keep the default (256): few cavities run at a time there, and
+# a large one capped at 16 workgroups would leave most CUs idle.
+MANY_CAVITY_MAXWG = "16"
+
+
 def run_batch(args):
+    os.environ.setdefault("PYFLOW_MAXWG", MANY_CAVITY_MAXWG)
     import numpy as np
     import torch
     from pyflow._device import DeviceBatch
@@ -571,7 +580,8 @@ def run_batch(args):
         "data": "synthetic: 256 lid-driven cavities from rest; n_i = 32*U{1..16}, Re_i log-uniform [100, 5000], "
                 "seed 0",
         "config": {"workload": f"256 cavities, QUICK, tol 1e-6, lin_maxiter 10000, {steps} steps", "cells": cells,
-                   "precision": args.precision, "inner_rtol": args.inner_rtol},
+                   "precision": args.precision, "inner_rtol": args.inner_rtol,
+                   "max_workgroups_per_cavity": int(os.environ["PYFLOW_MAXWG"])},
         "roofline": gpu_roofline("pf::k_round<float, U, false> (all cavities' regions in one launch)", alg_bytes,
                                  gpu_ms, rounds),
         "pcg": {"iterations_total": int(its.sum()), "mean_per_cavity_step": round(float(its.mean()) / steps, 1)},

@@ -902,13 +902,14 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
       const float u = minv_f32(C, A, j, i) * L.ro[m];
       pk = first ? u : (u + be * L.po[m]);
       const size_t k = (size_t)j * nx + i;
-      // Ordinary stores. Non-temporal (streaming) stores here let the next
-      // round's entry loads see stale values in multi-cavity batches.
-      Pn[k] = pk;
+      // Streaming stores: these lines are next read in the next round, by
+      // this and neighbouring workgroups, so keeping them dirty in L2 only
+      // adds to the write-back at the end of the kernel.
+      __builtin_nontemporal_store(pk, &Pn[k]);
       if constexpr (XD)
-        C.x64[k] = L.xo[m] + A.alpha * (double)pk;
+        __builtin_nontemporal_store(L.xo[m] + A.alpha * (double)pk, &C.x64[k]);
       else
-        X[k] = L.xo[m] + al * pk;
+        __builtin_nontemporal_store(L.xo[m] + al * pk, &X[k]);
     }
     lp[(TY * m + lj + 2) * HP + li + 2] = pk;
     pkr[m] = pk;
@@ -945,7 +946,7 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
         acc[4] += (double)uk1[m] * (double)sv;
         acc[5] += (double)rk1[m] * (double)pkr[m];
       }
-      Rn[(size_t)j * nx + i] = rk1[m];
+      __builtin_nontemporal_store(rk1[m], &Rn[(size_t)j * nx + i]);
     }
     lu[row * HP + li + 2] = uk1[m];
   }
