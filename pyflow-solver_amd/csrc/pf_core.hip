@@ -81,6 +81,7 @@ struct pf_batch {
   int total_blocks = 0;
   int single_slot = -1;  // the only used slot when the batch holds one cavity, else -1
   int max_tpb = 1;       // largest region height (tiles) over the batch's cavities
+  int lds_floats = 0;    // dynamic LDS of a float32 round: the largest PCG region's staging
   bool any_xd = false;   // some cavity keeps its PCG iterate in FP64
   bool multi = false;    // some cavity has several regions per workgroup
   bool tables_dirty = true;
@@ -123,6 +124,13 @@ static int env_int(const char* name, int dflt) {
   return x > 0 ? x : dflt;
 }
 
+// A switch: unset or empty -> dflt, else the integer value (0 = off).
+static int env_switch(const char* name, int dflt) {
+  const char* v = getenv(name);
+  if (!v || !*v) return dflt;
+  return atoi(v);
+}
+
 // Workgroup regions: M = tpb tiles (64 x 16 cells each) stacked in y, M a
 // power of two <= 8, starting from PYFLOW_TPB (default 4) and doubled while a
 // cavity would need more than PYFLOW_MAXBLK (default 1024) workgroups.
@@ -159,6 +167,12 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   return s;
 }
 
+// MIXED with an FP32 iterate: two PCG iterations per round (regions of up to
+// 4 tiles; their 4-cell frames fit in LDS). PYFLOW_CA=0 keeps one.
+static void choose_ca(PfCav& c, int prec) {
+  c.ca = (prec == PF_PREC_MIXED && !c.xd && c.tpb <= 4 && env_switch("PYFLOW_CA", 1) != 0) ? 1 : 0;
+}
+
 static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   c.nx = p->nx;
   c.ny = p->ny;
@@ -188,9 +202,10 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   // pass's starting residual; cond(A) ~ (2n/pi)^2 makes that 0.1 at n = 2048
   // and 0.4 at n = 4096, where the refinement passes stop contracting.
   // PYFLOW_XD=0/1 forces it off/on.
-  const int xd_env = env_int("PYFLOW_XD", -1);
+  const int xd_env = env_switch("PYFLOW_XD", -1);
   const int nmax = std::max(p->nx, p->ny);
   c.xd = (prec == PF_PREC_MIXED) ? (xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0)) : 0;
+  choose_ca(c, prec);
 }
 
 static int sync_tables(pf_batch* b) {
@@ -213,11 +228,14 @@ static int sync_tables(pf_batch* b) {
     }
   b->single_slot = (nused == 1) ? last : -1;
   b->max_tpb = 1;
+  b->lds_floats = 0;
   b->any_xd = false;
   b->multi = false;
   for (int s = 0; s < b->max_cav; ++s)
     if (b->used[s]) {
       b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
+      const int tpb = b->hcav[s].tpb;
+      b->lds_floats = std::max(b->lds_floats, b->hcav[s].ca ? ca_lds_floats(tpb) : region_lds_floats(tpb));
       b->any_xd = b->any_xd || b->hcav[s].xd;
       b->multi = b->multi || b->hcav[s].rpw > 1;
     }
@@ -264,7 +282,7 @@ static int check_slot(pf_batch* b, int slot) {
 template <class T, int U, bool MULTI>
 static void launch_round_m(pf_batch* b, const KArgs& K) {
   dim3 blk(TX, TY);
-  const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)region_lds_floats(b->max_tpb) * sizeof(float) : 0;
+  const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)b->lds_floats * sizeof(float) : 0;
   const dim3 grid(b->total_blocks);
   if (sizeof(T) == sizeof(float) && b->any_xd) {
     // FP64-iterate cavities are large (a round takes 100+ us): the descriptor
@@ -412,10 +430,11 @@ int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* pa
   return PF_OK;
 }
 
-// The float32 CG region of 8 tiles stages 2 x 132 x 68 floats (71.8 KB) in
-// LDS, above the 64 KB a launch may request without opting in.
+// The float32 PCG regions stage up to 5 x 72 x 72 floats (101 KiB; the
+// two-iteration round on 4 tiles) in LDS, above the 64 KiB a launch may
+// request without opting in.
 static int allow_region_lds() {
-  const int bytes = region_lds_floats(8) * (int)sizeof(float);
+  const int bytes = std::max(region_lds_floats(8), ca_lds_floats(4)) * (int)sizeof(float);
   const void* fns[6] = {
       (const void*)k_round<float, 2, true, false, false>, (const void*)k_round<float, 2, false, false, false>,
       (const void*)k_round<float, 2, false, true, false>, (const void*)k_round<float, 2, true, false, true>,
@@ -589,7 +608,15 @@ int pf_batch_set_params(pf_batch* b, int slot, const pf_cavity_params* params) {
   if (rc) return rc;
   PfCav& c = b->hcav[slot];
   if (params->nx != c.nx || params->ny != c.ny) return fail(PF_EINVAL, "nx/ny cannot change");
+  // The workspace (partials: 2 x nblk slots) was carved for the blocking
+  // chosen at creation; keep it whatever the environment says now.
+  const int nblk = c.nblk, tpb = c.tpb, nreg = c.nreg, rpw = c.rpw;
   fill_params(c, params, b->prec);
+  c.nblk = nblk;
+  c.tpb = tpb;
+  c.nreg = nreg;
+  c.rpw = rpw;
+  choose_ca(c, b->prec);
   b->tables_dirty = true;
   return PF_OK;
 }

@@ -31,9 +31,10 @@ constexpr int TX = 64;        // tile width = wavefront width
 constexpr int TY = 16;        // tile height; one wave per tile row
 constexpr int NTH = TX * TY;  // threads per workgroup
 constexpr int NWAVE = NTH / 64;
-constexpr int NP = 8;         // FP64 partial slots per workgroup (stride)
-constexpr int KMAX = 3;       // slots of every phase except the FP64-iterate PCG round
+constexpr int NP = 16;        // FP64 partial slots per workgroup (stride)
+constexpr int KMAX = 3;       // slots of every phase except the PCG rounds below
 constexpr int KXD = 6;        // slots of the FP64-iterate PCG round
+constexpr int KCA = 14;       // slots of the two-iteration PCG round
 
 enum Phase {
   PH_IDLE = 0,
@@ -49,7 +50,8 @@ enum Phase {
   PH_CORR_A = 10,
   PH_CORR_B = 11,
   PH_CG0 = 12,  // MIXED: w = A M r and the first dot products of an inner solve
-  PH_CG = 13,   // MIXED: one single-reduction (Chronopoulos-Gear) PCG iteration
+  PH_CG = 13,   // MIXED: one single-reduction (Chronopoulos-Gear) PCG iteration, or two
+                // s-step iterations on a cavity with PfCav::ca
   PH_DONE = 14,
 };
 
@@ -80,6 +82,7 @@ struct PfCav {
   int nreg, rpw;  // regions; regions per workgroup (workgroup b owns regions b*rpw .. b*rpw+rpw-1)
   int scheme, lid_on, jacobi, maxiter, op, mode, prec;
   int xd;  // MIXED: the PCG iterate is accumulated in FP64 (x64) instead of FP32
+  int ca;  // MIXED: two PCG iterations per round (s-step basis, 4-cell halo)
   double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
   void *u, *v, *p, *us, *vs, *r, *d0, *d1, *x;  // element type T
   void* r1;                                     // MIXED: r double-buffered (r, r1); p in d0/d1
@@ -93,9 +96,11 @@ struct PfScal {
   int pending;   // phase whose partials await their tail (0 = none)
   int steps_left, steps_done, it, it_total, info, outer, x_is_b;
   int hist_len, safeguard, nan, converged, it_run, outer_run, done_now;
+  int ca_inc, ca_pad;  // two-iteration round: iterations the pending round runs (0: basis only)
   double dt_prev, dt, cfl_now, umax, vmax, hist[5], div_before;
   double mean, div_initial, bnorm, atol, inner_atol, rho, rho_prev, beta, pq, alpha, rnorm2, rn_outer;
   double rnorm_true, div_after, ru, rv, continuity;
+  double ca0, cb0, ca1, cb1;  // two-iteration round: alpha_k, beta_k, alpha_k+1, beta_k+1
 };
 static_assert(sizeof(PfScal) % 4 == 0, "PfScal copied as 32-bit words");
 
@@ -228,6 +233,19 @@ __device__ __attribute__((always_inline)) inline void cg_next(const PfCav& C, Pf
   S.phase = mixed ? PH_CG : PH_P;
 }
 
+// First round of an inner FP32 solve: CG0, or on a two-iteration cavity a
+// round that runs no iteration and only returns the partials of the starting
+// basis (p = M r).
+__device__ inline void inner_start(const PfCav& C, PfScal& S) {
+  if (C.ca) {
+    S.phase = PH_CG;
+    S.ca_inc = 0;
+    S.ca0 = S.cb0 = S.ca1 = S.cb1 = 0.0;
+  } else {
+    S.phase = PH_CG0;
+  }
+}
+
 __device__ inline void enter_done(PfScal& S) {
   S.phase = PH_DONE;
   S.done_now = 1;
@@ -270,6 +288,74 @@ __device__ __attribute__((always_inline)) inline void tail_cg_xd(const PfCav& C,
   S.it_total += 1;
   S.it_run += 1;
   cg_next(C, S);
+}
+
+// Tail of a two-iteration PCG round (s-step PCG with s = 2: Chronopoulos &
+// Gear 1989; in the communication-avoiding form of Hoemmen 2010, ch. 5, and
+// Carson 2015). The round returned, for the new r, p and z = M r, the Gram
+// matrix W = Y^T M^-1 Y of the basis Y = [p, Bp, B^2 p, z, Bz], B = M A,
+// (13 entries: W01 W02 W04 W11 W12 W13 W14 W22 W23 W24 W33 W34 W44) and |r|^2.
+// Every quantity of the next two PCG iterations is a quadratic form in W:
+// with e_i the coordinate vectors of Y and T the shift (B Y e_i = Y T e_i):
+//   gamma_k = W33,                  alpha_k   = gamma_k / (e0.W.T e0)
+//   z_k+1 = e3 - alpha_k e1,        gamma_k+1 = z_k+1.W.z_k+1,  beta_k = gamma_k+1/gamma_k
+//   p_k+1 = z_k+1 + beta_k e0,      alpha_k+1 = gamma_k+1 / (p_k+1.W.T p_k+1)
+//   z_k+2 = z_k+1 - alpha_k+1 T p_k+1,  beta_k+1 = z_k+2.W.z_k+2 / gamma_k+1
+// W is a weighted Gram matrix, symmetric whatever A is, so the identity rows
+// of the pinned and Dirichlet operators need no special case.
+__device__ __attribute__((always_inline)) inline void tail_ca(const PfCav& C, PfScal& S, const double* t) {
+  S.pending = PH_IDLE;
+  const int inc = S.ca_inc;
+  S.it += inc;
+  S.it_total += inc;
+  S.it_run += inc;
+  S.rnorm2 = t[13];
+  S.rho = t[10];
+  if (S.it_total >= C.maxiter) {
+    S.info = C.maxiter;
+    S.phase = outer_entry(C);
+    return;
+  }
+  if (sqrt(S.rnorm2) < S.inner_atol) {
+    S.phase = outer_entry(C);
+    return;
+  }
+  const double w01 = t[0], w02 = t[1], w04 = t[2], w11 = t[3], w12 = t[4], w13 = t[5], w14 = t[6];
+  const double w22 = t[7], w23 = t[8], w24 = t[9], w33 = t[10], w34 = t[11], w44 = t[12];
+  double a0 = w33 / w01;
+  double b0 = 0.0, a1 = 0.0, b1 = 0.0;
+  int n = 1;
+  // alpha < 0 is legitimate: the Dirichlet operator (+lap) is negative
+  // definite. gamma = r.M r > 0 whatever the sign of A.
+  if (a0 == 0.0 || !isfinite(a0)) {
+    // No usable step (only reachable with corrupted input): count one
+    // iteration and restart the direction at p = M r.
+    a0 = 0.0;
+  } else {
+    const double g1 = w33 - 2.0 * a0 * w13 + a0 * a0 * w11;
+    b0 = g1 / w33;
+    const double pap = (w34 - a0 * w23 + b0 * w13) + (-a0 * w14 + a0 * a0 * w12 - a0 * b0 * w11) +
+                       (b0 * w04 - a0 * b0 * w02 + b0 * b0 * w01);
+    const double a1c = g1 / pap;
+    if (C.maxiter - S.it_total >= 2 && g1 > 0.0 && pap != 0.0 && isfinite(a1c)) {
+      a1 = a1c;
+      n = 2;
+      const double k1 = -(a0 + a1 * b0), k2 = a1 * a0, k4 = -a1;
+      const double g2 = w33 + k1 * k1 * w11 + k2 * k2 * w22 + k4 * k4 * w44 +
+                        2.0 * (k1 * w13 + k2 * w23 + k4 * w34 + k1 * k2 * w12 + k1 * k4 * w14 + k2 * k4 * w24);
+      b1 = g2 / g1;
+      if (!isfinite(b1)) b1 = 0.0;
+    }
+    if (!isfinite(b0)) b0 = 0.0;
+  }
+  S.ca0 = a0;
+  S.cb0 = b0;
+  S.ca1 = a1;
+  S.cb1 = b1;
+  S.ca_inc = n;
+  S.alpha = n == 2 ? a1 : a0;
+  S.beta = n == 2 ? b1 : b0;
+  S.phase = PH_CG;
 }
 
 __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
@@ -346,7 +432,7 @@ __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
           break;
         }
         S.inner_atol = pymax(C.inner_rtol * bn, 0.5 * S.atol);
-        S.phase = PH_CG0;
+        inner_start(C, S);
         break;
       }
       S.inner_atol = S.atol;
@@ -385,7 +471,7 @@ __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
       }
       S.it = 0;
       S.inner_atol = pymax(C.inner_rtol * rn, 0.5 * S.atol);
-      S.phase = PH_CG0;
+      inner_start(C, S);
       break;
     }
     case PH_CG0:  // gamma0 = r.Mr, delta0 = (A M r).(M r), alpha0 = gamma0/delta0
@@ -442,6 +528,8 @@ struct PhaseArgs {
   double dt, mean, beta, alpha, x0;
   int it, x_is_b, par;  // par = round parity
   float cx, cy, betaf, alphaf, minvf;  // float32 path: 1/dx^2, 1/dy^2, CG scalars, Jacobi weight
+  float ca0f, cb0f, ca1f, cb1f;        // two-iteration round: alpha_k, beta_k, alpha_k+1, beta_k+1
+  int ca_init;                         // two-iteration round that only forms the starting basis
 };
 
 // ---------------------------------------------------------------------------
@@ -984,6 +1072,265 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two PCG iterations per round (cavities with PfCav::ca; the tail is tail_ca).
+// A workgroup region of 64 x RH cells is staged in LDS with a 4-cell frame:
+//   pass 1  loads r_k, p_k at the cells and the frame, x at the cells;
+//           p (0 in the basis-only round) and z = M r into LDS;
+//   pass 2  a = A p within distance 3; M a into LDS;
+//   pass 3  c = A M a and e = A z within distance 2, then the two steps
+//             p_k+1 = z - alpha_k M a + beta_k p,  A p_k+1 = e - alpha_k c + beta_k a
+//             r_k+2 = r - alpha_k a - alpha_k+1 A p_k+1,  z_k+2 = M r_k+2
+//             p_k+2 = z_k+2 + beta_k+1 p_k+1,  x_k+2 = x + alpha_k p + alpha_k+1 p_k+1;
+//           stores r_k+2, p_k+2, x_k+2; p_k+2, z_k+2 into LDS;
+//   pass 4  a' = A p_k+2 within distance 1; M a' into LDS;
+//   pass 5  c' = A M a', e' = A z_k+2 and the Gram partials at the cells.
+// The frame cells are recomputed from the same inputs with the same
+// expressions as their owners compute them, so they carry the same values.
+// HBM traffic per cell is that of the one-iteration round (read r, p, x;
+// write r, p, x) for two iterations; the frame re-reads hit L2.
+constexpr int HQ = TX + 8;  // LDS row pitch: 64 columns + 4-cell frame each side
+__host__ __device__ constexpr int ca_lds_floats(int m) { return 5 * (TY * m + 8) * HQ; }
+// Cells at Chebyshev distance exactly d from a region of rh rows, and within D.
+__host__ __device__ constexpr int ca_ring_cells(int rh, int d) { return 2 * (TX + 2 * d) + 2 * (rh + 2 * d - 2); }
+__host__ __device__ constexpr int ca_frame_cells(int rh, int D) {
+  return D <= 0 ? 0 : ca_frame_cells(rh, D - 1) + ca_ring_cells(rh, D);
+}
+
+// LDS row/column of frame cell q, frame cells numbered ring by ring (distance
+// 1, 2, 3, 4; each ring: top row, bottom row, left column, right column), so
+// the cells within distance D are exactly q < ca_frame_cells(rh, D). Region
+// cell (jj, ii) sits at row jj + 4, column ii + 4.
+__host__ __device__ inline void ca_frame_cell(int rh, int q, int& row, int& col) {
+  int d = 1;
+  const int n1 = ca_ring_cells(rh, 1), n2 = ca_ring_cells(rh, 2), n3 = ca_ring_cells(rh, 3);
+  if (q >= n1) {
+    q -= n1;
+    d = 2;
+    if (q >= n2) {
+      q -= n2;
+      d = 3;
+      if (q >= n3) {
+        q -= n3;
+        d = 4;
+      }
+    }
+  }
+  const int w = TX + 2 * d, hs = rh + 2 * d - 2;
+  if (q < w) {
+    row = 4 - d;
+    col = 4 - d + q;
+  } else if (q < 2 * w) {
+    row = 3 + rh + d;
+    col = 4 - d + (q - w);
+  } else if (q < 2 * w + hs) {
+    col = 4 - d;
+    row = 5 - d + (q - 2 * w);
+  } else {
+    col = 3 + TX + d;
+    row = 5 - d + (q - 2 * w - hs);
+  }
+}
+
+// The two steps at one cell (pass 3), shared by cells and frame.
+struct CaCell {
+  float p1, r2, z2, p2;
+};
+__device__ __attribute__((always_inline)) inline CaCell ca_steps(const PhaseArgs& A, float m, float r, float p, float a,
+                                                                 float c, float e) {
+  CaCell o;
+  const float z = m * r;
+  const float ma = m * a;
+  o.p1 = (z - A.ca0f * ma) + A.cb0f * p;
+  const float ap1 = (e - A.ca0f * c) + A.cb0f * a;
+  o.r2 = (r - A.ca0f * a) - A.ca1f * ap1;
+  o.z2 = m * o.r2;
+  o.p2 = o.z2 + A.cb1f * o.p1;
+  return o;
+}
+
+template <int M>
+__device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, const PhaseArgs& A, int r, double* acc,
+                                                                float* __restrict__ lds) {
+  const int nx = C.nx, ny = C.ny;
+  constexpr int RH = TY * M;
+  constexpr int NR = RH + 8;
+  constexpr int F1 = ca_frame_cells(RH, 1), F2 = ca_frame_cells(RH, 2), F3 = ca_frame_cells(RH, 3);
+  constexpr int F4 = ca_frame_cells(RH, 4);
+  constexpr int NS = (F4 + NTH - 1) / NTH;  // frame slots per thread; slots past the first are loads only
+  static_assert(F3 <= NTH, "one thread per frame cell within distance 3");
+  static_assert(NR * HQ * 5 == ca_lds_floats(M), "LDS layout");
+  float* __restrict__ lP = lds;                 // p_k
+  float* __restrict__ lZ = lds + NR * HQ;       // z_k = M r_k
+  float* __restrict__ lMA = lds + 2 * NR * HQ;  // M A p_k, then M A p_k+2
+  float* __restrict__ lP2 = lds + 3 * NR * HQ;  // p_k+2
+  float* __restrict__ lZ2 = lds + 4 * NR * HQ;  // z_k+2
+  // Buffers follow the round parity q as in region_cg_f32: read r[q], p[q^1],
+  // write r[q^1], p[q]. The basis-only round runs the round after PINIT /
+  // OUTER_B, which wrote r[q^1]: it reads r there, leaves r and x as they
+  // are (the next round reads r[q^1] again) and writes p = M r to p[q].
+  const bool init = A.ca_init != 0;
+  const float* __restrict__ R = mixed_r(C, init ? (A.par ^ 1) : A.par);
+  const float* __restrict__ Pold = mixed_p(C, A.par ^ 1);
+  float* __restrict__ Rn = mixed_r(C, A.par ^ 1);
+  float* __restrict__ Pn = mixed_p(C, A.par);
+  float* __restrict__ X = (float*)C.x;
+  const int li = threadIdx.x, lj = threadIdx.y;
+  const int tid = lj * TX + li;
+  const int i0 = (r % C.tiles_x) * TX;
+  const int j0 = (r / C.tiles_x) * RH;
+  const int i = i0 + li;
+  auto op_at = [&](const float* L, int row, int col, int gj, int gi) -> float {
+    const float* q = L + row * HQ + col;
+    return apply_op5_fast(C.op, q[0], q[1], q[-1], q[HQ], q[-HQ], nx, ny, gj, gi, A.cx, A.cy);
+  };
+  // Pass 1: every global load of the thread, unconditional (out-of-range
+  // lanes read cell 0 and discard it).
+  float ro[M], po[M], xo[M];
+  bool in[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    in[m] = (i < nx) && (j < ny);
+    const size_t k = in[m] ? (size_t)j * nx + i : 0;
+    const float rv = R[k], pv = Pold[k], xv = X[k];
+    ro[m] = in[m] ? rv : 0.f;
+    po[m] = (in[m] && !init) ? pv : 0.f;
+    xo[m] = in[m] ? xv : 0.f;
+  }
+  float fr[NS], fp[NS];
+  int frow[NS], fcol[NS];
+  bool fin[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int q = tid + s * NTH;
+    int row = 0, col = 0;
+    if (q < F4) ca_frame_cell(RH, q, row, col);
+    frow[s] = row;
+    fcol[s] = col;
+    const int gj = j0 + row - 4, gi = i0 + col - 4;
+    fin[s] = (q < F4) && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+    const size_t kk = fin[s] ? (size_t)gj * nx + gi : 0;
+    const float rv = R[kk], pv = Pold[kk];
+    fr[s] = fin[s] ? rv : 0.f;
+    fp[s] = (fin[s] && !init) ? pv : 0.f;
+  }
+  float mo[M];  // Jacobi weights of the cells
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    mo[m] = in[m] ? minv_f32(C, A, j, i) : 0.f;
+    const int o = (TY * m + lj + 4) * HQ + li + 4;
+    lP[o] = po[m];
+    lZ[o] = mo[m] * ro[m];
+  }
+  float fm[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int q = tid + s * NTH;
+    fm[s] = fin[s] ? minv_f32(C, A, j0 + frow[s] - 4, i0 + fcol[s] - 4) : 0.f;
+    if (q < F4) {
+      const int o = frow[s] * HQ + fcol[s];
+      lP[o] = fp[s];
+      lZ[o] = fm[s] * fr[s];
+    }
+  }
+  __syncthreads();
+  // Pass 2: a = A p.
+  float ao[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    const int row = TY * m + lj + 4;
+    ao[m] = in[m] ? op_at(lP, row, li + 4, j, i) : 0.f;
+    lMA[row * HQ + li + 4] = mo[m] * ao[m];
+  }
+  float fa = 0.f;
+  if (tid < F3) {
+    fa = fin[0] ? op_at(lP, frow[0], fcol[0], j0 + frow[0] - 4, i0 + fcol[0] - 4) : 0.f;
+    lMA[frow[0] * HQ + fcol[0]] = fm[0] * fa;
+  }
+  __syncthreads();
+  // Pass 3: the two steps.
+  float r2o[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    const int row = TY * m + lj + 4;
+    const int o = row * HQ + li + 4;
+    CaCell v{0.f, 0.f, 0.f, 0.f};
+    if (in[m]) {
+      const float cv = op_at(lMA, row, li + 4, j, i);
+      const float ev = op_at(lZ, row, li + 4, j, i);
+      v = ca_steps(A, mo[m], ro[m], po[m], ao[m], cv, ev);
+      const size_t k = (size_t)j * nx + i;
+      // Streaming stores: the lines are next read in the next round.
+      __builtin_nontemporal_store(v.p2, &Pn[k]);
+      if (!init) {
+        __builtin_nontemporal_store(v.r2, &Rn[k]);
+        __builtin_nontemporal_store((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1, &X[k]);
+      }
+    }
+    r2o[m] = v.r2;
+    lP2[o] = v.p2;
+    lZ2[o] = v.z2;
+  }
+  if (tid < F2) {
+    CaCell v{0.f, 0.f, 0.f, 0.f};
+    if (fin[0]) {
+      const int gj = j0 + frow[0] - 4, gi = i0 + fcol[0] - 4;
+      const float cv = op_at(lMA, frow[0], fcol[0], gj, gi);
+      const float ev = op_at(lZ, frow[0], fcol[0], gj, gi);
+      v = ca_steps(A, fm[0], fr[0], fp[0], fa, cv, ev);
+    }
+    const int o = frow[0] * HQ + fcol[0];
+    lP2[o] = v.p2;
+    lZ2[o] = v.z2;
+  }
+  __syncthreads();
+  // Pass 4: a' = A p_k+2 (lMA is rewritten: pass 3's reads of it are done).
+  float a2o[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    const int row = TY * m + lj + 4;
+    a2o[m] = in[m] ? op_at(lP2, row, li + 4, j, i) : 0.f;
+    lMA[row * HQ + li + 4] = mo[m] * a2o[m];
+  }
+  if (tid < F1) {
+    const float a2 = fin[0] ? op_at(lP2, frow[0], fcol[0], j0 + frow[0] - 4, i0 + fcol[0] - 4) : 0.f;
+    lMA[frow[0] * HQ + fcol[0]] = fm[0] * a2;
+  }
+  __syncthreads();
+  // Pass 5: partials of the new basis at the cells.
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    if (in[m]) {
+      const int j = j0 + TY * m + lj;
+      const int row = TY * m + lj + 4;
+      const int o = row * HQ + li + 4;
+      const float cv = op_at(lMA, row, li + 4, j, i);
+      const float ev = op_at(lZ2, row, li + 4, j, i);
+      const double p2 = lP2[o], z2 = lZ2[o], r2 = r2o[m], a2 = a2o[m];
+      const double ma2 = mo[m] * a2o[m], c2 = cv, mc2 = mo[m] * cv, e2 = ev, me2 = mo[m] * ev;
+      acc[0] += p2 * a2;     // W01
+      acc[1] += p2 * c2;     // W02
+      acc[2] += p2 * e2;     // W04
+      acc[3] += a2 * ma2;    // W11
+      acc[4] += a2 * mc2;    // W12
+      acc[5] += a2 * z2;     // W13
+      acc[6] += a2 * me2;    // W14
+      acc[7] += c2 * mc2;    // W22
+      acc[8] += c2 * z2;     // W23
+      acc[9] += c2 * me2;    // W24
+      acc[10] += r2 * z2;    // W33
+      acc[11] += z2 * e2;    // W34
+      acc[12] += e2 * me2;   // W44
+      acc[13] += r2 * r2;    // |r|^2
+    }
+  }
+}
+
 template <class T>
 __device__ inline void cell_phase(int phase, const PfCav& C, const PhaseArgs& A, int j, int i, double* acc) {
   switch (phase) {
@@ -1190,6 +1537,113 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   }
 }
 
+// Round body of a two-iteration cavity (PfCav::ca). The pending phase's
+// partials are folded by one wave per slot: wave w sums slot w of every
+// workgroup (per lane: increasing workgroup index; then the DPP tree), so all
+// fourteen slots of the PCG round fold in parallel; thread 63 runs the tail.
+template <int M, bool MULTI>
+__device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c,
+                                                                    int b, PfScal& st, double (*sm)[NP], double* tot,
+                                                                    float* lds) {
+  const int parity = KA.parity;
+  const int tid = tid_flat();
+  const uint32_t* src = (const uint32_t*)(KA.scal + (size_t)(parity ^ 1) * KA.max_cav + c);
+  uint32_t* dst = (uint32_t*)(KA.scal + (size_t)parity * KA.max_cav + c);
+  constexpr int NW32 = (int)(sizeof(PfScal) / 4);
+  static_assert(NW32 <= NTH, "state copied with one word per thread");
+  static_assert(NWAVE == NP && KCA <= NP, "one folding wave per partial slot");
+  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
+  const int nblk = C.nblk;
+  const int lane = tid & 63, wave = tid >> 6;
+  constexpr int PRE = 4;
+  double pre[PRE];
+#pragma unroll
+  for (int t = 0; t < PRE; ++t) {
+    const int q = lane + 64 * t;
+    pre[t] = prev[(size_t)(q < nblk ? q : 0) * NP + wave];
+  }
+  const uint32_t sw = src[tid < NW32 ? tid : 0];
+  if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
+  __syncthreads();
+  if (st.pending != PH_IDLE) {
+    const int pend = st.pending;
+    const bool hot = pend == PH_CG;
+    const int K = hot ? KCA : phase_nparts(pend);
+    if (wave < K) {
+      const bool mx = !hot && ((phase_maxmask(pend) >> wave) & 1u);
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < PRE; ++t)
+        if (lane + 64 * t < nblk) v = mx ? nanmax(v, pre[t]) : v + pre[t];
+      for (int q = lane + 64 * PRE; q < nblk; q += 64) {
+        const double w = prev[(size_t)q * NP + wave];
+        v = mx ? nanmax(v, w) : v + w;
+      }
+      v = mx ? wave_reduce<true>(v) : wave_reduce<false>(v);
+      if (lane == 63) tot[wave] = v;
+    }
+    __syncthreads();  // every slot's total visible to thread 63
+    if (tid == 63) {
+      if (__builtin_expect(hot, 1))
+        tail_ca(C, st, tot);
+      else
+        tail(*Cg, st, tot);
+    }
+    __syncthreads();  // tail result visible
+  }
+  const int phase = st.phase;
+  if (phase <= PH_IDLE || phase >= PH_DONE) {
+    if (b == 0) {
+      const int newly_done = st.done_now;
+      __syncthreads();
+      if (tid == 0) st.done_now = 0;
+      __syncthreads();
+      if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+      if (tid == 0 && newly_done) atomicSub(KA.active, 1);
+    }
+    return;
+  }
+  PhaseArgs A;
+  A.dt = st.dt;
+  A.mean = st.mean;
+  A.beta = st.beta;
+  A.alpha = st.alpha;
+  A.it = st.it;
+  A.x_is_b = st.x_is_b;
+  A.par = parity;
+  A.x0 = 0.0;
+  if (phase == PH_FIN) A.x0 = st.x_is_b ? C.b[0] : C.x64[0];
+  A.cx = (float)(1.0 / (C.dx * C.dx));
+  A.cy = (float)(1.0 / (C.dy * C.dy));
+  A.betaf = (float)A.beta;
+  A.alphaf = (float)A.alpha;
+  A.minvf = (float)C.invd;
+  A.ca0f = (float)st.ca0;
+  A.cb0f = (float)st.cb0;
+  A.ca1f = (float)st.ca1;
+  A.cb1f = (float)st.cb1;
+  A.ca_init = (st.ca_inc == 0);
+  double acc[NP] = {0.0, 0.0, 0.0, 0.0};
+  double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
+  const int r0 = MULTI ? b * C.rpw : b;
+  const int r1 = MULTI ? min(r0 + C.rpw, C.nreg) : b + 1;
+  if (__builtin_expect(phase == PH_CG, 1)) {
+    for (int r = r0; r < r1; ++r) {
+      if (r > r0) __syncthreads();  // the previous region's LDS reads are done
+      region_ca<M>(C, A, r, acc, lds);
+    }
+    block_partials_k<KCA, 0u>(acc, sm, part_out);
+  } else {
+    round_cold<float>(phase, Cg, A, b, sm, part_out);
+  }
+  if (b == 0) {
+    __syncthreads();
+    if (tid == 0) st.pending = phase;
+    __syncthreads();
+    if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+  }
+}
+
 // XDANY: some cavity of the batch keeps its PCG iterate in FP64; otherwise
 // only the FP32-iterate body is compiled, which keeps the kernel small.
 // MULTI: some cavity has several regions per workgroup.
@@ -1198,7 +1652,7 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   __shared__ PfScal st;
   __shared__ double sm[NWAVE][NP];
   __shared__ double tot[NP];
-  extern __shared__ float lds_dyn[];  // region_lds_floats(tpb) floats (float32 CG region)
+  extern __shared__ float lds_dyn[];  // float32 PCG region: region_lds_floats(tpb), or ca_lds_floats(tpb) floats
   const int g = blockIdx.x;
   int c;
   const PfCav* cp;
@@ -1227,6 +1681,12 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
         case 2: round_body<T, U, 2, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
         case 4: round_body<T, U, 4, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
         default: round_body<T, U, 8, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      }
+    } else if (C.ca) {
+      switch (C.tpb) {
+        case 1: round_body_ca<1, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body_ca<2, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body_ca<4, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
       }
     } else {
       switch (C.tpb) {

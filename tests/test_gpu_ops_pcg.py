@@ -145,3 +145,49 @@ def test_device_cg_mixed_fp64_iterate(n, monkeypatch):
         x, info, its, rnorm, conv = device_cg(op, b, tol, 20000, jacobi=True, precision="mixed")
         assert info == 0 and conv
         assert float(np.linalg.norm(b - mv(x))) <= tol * np.linalg.norm(b) * 1.0001
+
+
+@pytest.mark.parametrize("n", [64, 200])
+def test_device_cg_two_iterations_per_round(n, monkeypatch):
+    """The two-iteration (s = 2) PCG round, the default for FP32-iterate
+    cavities, meets the FP64 stopping rule on the true residual like the
+    one-iteration round (PYFLOW_CA=0), on the pinned and the Dirichlet
+    operator, with and without the Jacobi preconditioner, in at most a few
+    more iterations; an odd iteration cap is honoured exactly."""
+    from pyflow.linear_solvers.interface import DirichletPosLaplacian, PinnedNegLaplacian, device_cg
+    b = _rhs(n, n, 5)
+    h = 1.0 / (n - 1)
+    # The Dirichlet operator is the Laplacian inside and the identity on the
+    # boundary; as in the reference's manufactured mode the right-hand side is
+    # zero on the boundary, which keeps the solve on the definite interior block.
+    bd = b.reshape(n, n).copy()
+    bd[0, :] = bd[-1, :] = bd[:, 0] = bd[:, -1] = 0.0
+    cases = [
+        ("pinned", PinnedNegLaplacian(n, n, 1.0, 1.0), lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0), True, b),
+        ("pinned-noM", PinnedNegLaplacian(n, n, 1.0, 1.0), lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0),
+         False, b),
+        ("dirichlet", DirichletPosLaplacian(n, n, h, h), lambda z: O.pos_laplacian_dirichlet(z, n, n, h, h), True,
+         bd.reshape(-1)),
+    ]
+    out, bad = {}, []
+    for name, op, mv, jac, b in cases:
+        for ca in ("1", "0"):
+            monkeypatch.setenv("PYFLOW_CA", ca)
+            for tol in (1e-6, 1e-10):
+                x, info, it, rnorm, conv = device_cg(op, b, tol, 40000, jacobi=jac, precision="mixed")
+                rel = float(np.linalg.norm(b - mv(x))) / np.linalg.norm(b)
+                print(f"[ca] n={n} op={name} ca={ca} tol={tol:g}: info={info} its={it} rel={rel:.3e}", flush=True)
+                out[name, ca, tol] = it
+                if not (info == 0 and conv and rel <= tol * 1.0001):
+                    bad.append((name, ca, tol, info, it, rel))
+    name, op, mv, jac, b = cases[2]
+    x, info, it, rnorm, conv = device_cg(op, b, 1e-6, 40000, jacobi=jac, precision="fp64")
+    print(f"[ca] n={n} op={name} fp64 tol=1e-06: info={info} its={it}", flush=True)
+    assert not bad, bad
+    for name, *_ in cases:
+        for tol in (1e-6, 1e-10):
+            assert out[name, "1", tol] <= 1.2 * out[name, "0", tol] + 4, (name, tol, out)
+    monkeypatch.setenv("PYFLOW_CA", "1")
+    op, b = cases[0][1], cases[0][4]
+    x, info, it, rnorm, conv = device_cg(op, b, 1e-14, 7, jacobi=True, precision="mixed")
+    assert info == 7 and it == 7 and not conv
