@@ -1180,9 +1180,39 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
   const int i0 = (r % C.tiles_x) * TX;
   const int j0 = (r / C.tiles_x) * RH;
   const int i = i0 + li;
-  auto op_at = [&](const float* L, int row, int col, int gj, int gi) -> float {
-    const float* q = L + row * HQ + col;
-    return apply_op5_fast(C.op, q[0], q[1], q[-1], q[HQ], q[-HQ], nx, ny, gj, gi, A.cx, A.cy);
+  // Operator without boundary tests. A cell on the domain boundary also
+  // writes its value into the LDS ghost across that boundary, so the missing
+  // neighbour's difference (e - c etc.) is exactly +0, as the reflective
+  // stencil drops it (lap5_fast); the operator's sign is folded into
+  // 1/dx^2, 1/dy^2 (round-to-nearest is sign-symmetric). Identity rows (the
+  // pinned cell; the Dirichlet boundary) return the cell's value. Cells
+  // outside the domain are never written and never read.
+  const bool pinned = (C.op == 0);
+  const float scx = pinned ? -A.cx : A.cx, scy = pinned ? -A.cy : A.cy;
+  // Cell code: LDS offsets of the (at most two) ghosts the cell writes, the
+  // cell's own offset standing in for a missing one (rewriting the same value
+  // there is harmless), so every write is three plain stores; bit 31 marks an
+  // identity row.
+  auto code_of = [&](int gj, int gi, int row, int col) -> unsigned {
+    const int o = row * HQ + col;
+    int g1 = o, g2 = o;
+    if (gi == nx - 1 && col + 1 < HQ) g1 = o + 1;
+    if (gi == 0 && col >= 1) g1 = o - 1;
+    if (gj == ny - 1 && row + 1 < NR) g2 = o + HQ;
+    if (gj == 0 && row >= 1) g2 = o - HQ;
+    const bool id = pinned ? (gj == 0 && gi == 0) : (gj == 0 || gj == ny - 1 || gi == 0 || gi == nx - 1);
+    return (unsigned)g1 | ((unsigned)g2 << 15) | (id ? 0x80000000u : 0u);
+  };
+  static_assert(NR * HQ < (1 << 15), "LDS offsets fit 15 bits");
+  auto put = [&](float* L, int o, unsigned code, float v) {
+    L[o] = v;
+    L[code & 0x7fffu] = v;
+    L[(code >> 15) & 0x7fffu] = v;
+  };
+  auto opv = [&](const float* L, int o, unsigned code) -> float {
+    const float c = L[o];
+    const float t = __fmul_rn((L[o + 1] - c) - (c - L[o - 1]), scx) + __fmul_rn((L[o + HQ] - c) - (c - L[o - HQ]), scy);
+    return (code & 0x80000000u) ? c : t;
   };
   // Pass 1: every global load of the thread, unconditional (out-of-range
   // lanes read cell 0 and discard it).
@@ -1199,40 +1229,45 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
     xo[m] = in[m] ? xv : 0.f;
   }
   float fr[NS], fp[NS];
-  int frow[NS], fcol[NS];
+  int fo[NS];
+  unsigned fc[NS];
   bool fin[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const int q = tid + s * NTH;
     int row = 0, col = 0;
     if (q < F4) ca_frame_cell(RH, q, row, col);
-    frow[s] = row;
-    fcol[s] = col;
+    fo[s] = row * HQ + col;
     const int gj = j0 + row - 4, gi = i0 + col - 4;
     fin[s] = (q < F4) && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+    fc[s] = fin[s] ? code_of(gj, gi, row, col) : 0;
     const size_t kk = fin[s] ? (size_t)gj * nx + gi : 0;
     const float rv = R[kk], pv = Pold[kk];
     fr[s] = fin[s] ? rv : 0.f;
     fp[s] = (fin[s] && !init) ? pv : 0.f;
   }
-  float mo[M];  // Jacobi weights of the cells
+  const int o0 = (lj + 4) * HQ + li + 4;  // LDS offset of the thread's first cell; cell m at o0 + m*TY*HQ
+  float mo[M];     // Jacobi weights of the cells
+  unsigned cc[M];  // ghost / identity codes of the cells
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = j0 + TY * m + lj;
     mo[m] = in[m] ? minv_f32(C, A, j, i) : 0.f;
-    const int o = (TY * m + lj + 4) * HQ + li + 4;
-    lP[o] = po[m];
-    lZ[o] = mo[m] * ro[m];
+    cc[m] = in[m] ? code_of(j, i, TY * m + lj + 4, li + 4) : 0;
+    if (in[m]) {
+      put(lP, o0 + m * TY * HQ, cc[m], po[m]);
+      put(lZ, o0 + m * TY * HQ, cc[m], mo[m] * ro[m]);
+    }
   }
-  float fm[NS];
+  float fm0 = 0.f;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const int q = tid + s * NTH;
-    fm[s] = fin[s] ? minv_f32(C, A, j0 + frow[s] - 4, i0 + fcol[s] - 4) : 0.f;
-    if (q < F4) {
-      const int o = frow[s] * HQ + fcol[s];
-      lP[o] = fp[s];
-      lZ[o] = fm[s] * fr[s];
+    if (fin[s]) {
+      const int row = fo[s] / HQ, col = fo[s] - row * HQ;
+      const float fm = minv_f32(C, A, j0 + row - 4, i0 + col - 4);
+      if (s == 0) fm0 = fm;
+      put(lP, fo[s], fc[s], fp[s]);
+      put(lZ, fo[s], fc[s], fm * fr[s]);
     }
   }
   __syncthreads();
@@ -1240,94 +1275,154 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
   float ao[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const int j = j0 + TY * m + lj;
-    const int row = TY * m + lj + 4;
-    ao[m] = in[m] ? op_at(lP, row, li + 4, j, i) : 0.f;
-    lMA[row * HQ + li + 4] = mo[m] * ao[m];
+    ao[m] = 0.f;
+    if (in[m]) {
+      ao[m] = opv(lP, o0 + m * TY * HQ, cc[m]);
+      put(lMA, o0 + m * TY * HQ, cc[m], mo[m] * ao[m]);
+    }
   }
   float fa = 0.f;
-  if (tid < F3) {
-    fa = fin[0] ? op_at(lP, frow[0], fcol[0], j0 + frow[0] - 4, i0 + fcol[0] - 4) : 0.f;
-    lMA[frow[0] * HQ + fcol[0]] = fm[0] * fa;
+  if (tid < F3 && fin[0]) {
+    fa = opv(lP, fo[0], fc[0]);
+    put(lMA, fo[0], fc[0], fm0 * fa);
   }
   __syncthreads();
   // Pass 3: the two steps.
   float r2o[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const int j = j0 + TY * m + lj;
-    const int row = TY * m + lj + 4;
-    const int o = row * HQ + li + 4;
-    CaCell v{0.f, 0.f, 0.f, 0.f};
+    r2o[m] = 0.f;
     if (in[m]) {
-      const float cv = op_at(lMA, row, li + 4, j, i);
-      const float ev = op_at(lZ, row, li + 4, j, i);
-      v = ca_steps(A, mo[m], ro[m], po[m], ao[m], cv, ev);
-      const size_t k = (size_t)j * nx + i;
+      const int o = o0 + m * TY * HQ;
+      const float cv = opv(lMA, o, cc[m]);
+      const float ev = opv(lZ, o, cc[m]);
+      const CaCell v = ca_steps(A, mo[m], ro[m], po[m], ao[m], cv, ev);
+      const size_t k = (size_t)(j0 + TY * m + lj) * nx + i;
       // Streaming stores: the lines are next read in the next round.
       __builtin_nontemporal_store(v.p2, &Pn[k]);
       if (!init) {
         __builtin_nontemporal_store(v.r2, &Rn[k]);
         __builtin_nontemporal_store((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1, &X[k]);
       }
+      r2o[m] = v.r2;
+      put(lP2, o, cc[m], v.p2);
+      put(lZ2, o, cc[m], v.z2);
     }
-    r2o[m] = v.r2;
-    lP2[o] = v.p2;
-    lZ2[o] = v.z2;
   }
-  if (tid < F2) {
-    CaCell v{0.f, 0.f, 0.f, 0.f};
-    if (fin[0]) {
-      const int gj = j0 + frow[0] - 4, gi = i0 + fcol[0] - 4;
-      const float cv = op_at(lMA, frow[0], fcol[0], gj, gi);
-      const float ev = op_at(lZ, frow[0], fcol[0], gj, gi);
-      v = ca_steps(A, fm[0], fr[0], fp[0], fa, cv, ev);
-    }
-    const int o = frow[0] * HQ + fcol[0];
-    lP2[o] = v.p2;
-    lZ2[o] = v.z2;
+  if (tid < F2 && fin[0]) {
+    const float cv = opv(lMA, fo[0], fc[0]);
+    const float ev = opv(lZ, fo[0], fc[0]);
+    const CaCell v = ca_steps(A, fm0, fr[0], fp[0], fa, cv, ev);
+    put(lP2, fo[0], fc[0], v.p2);
+    put(lZ2, fo[0], fc[0], v.z2);
   }
   __syncthreads();
   // Pass 4: a' = A p_k+2 (lMA is rewritten: pass 3's reads of it are done).
   float a2o[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const int j = j0 + TY * m + lj;
-    const int row = TY * m + lj + 4;
-    a2o[m] = in[m] ? op_at(lP2, row, li + 4, j, i) : 0.f;
-    lMA[row * HQ + li + 4] = mo[m] * a2o[m];
+    a2o[m] = 0.f;
+    if (in[m]) {
+      a2o[m] = opv(lP2, o0 + m * TY * HQ, cc[m]);
+      put(lMA, o0 + m * TY * HQ, cc[m], mo[m] * a2o[m]);
+    }
   }
-  if (tid < F1) {
-    const float a2 = fin[0] ? op_at(lP2, frow[0], fcol[0], j0 + frow[0] - 4, i0 + fcol[0] - 4) : 0.f;
-    lMA[frow[0] * HQ + fcol[0]] = fm[0] * a2;
-  }
+  if (tid < F1 && fin[0]) put(lMA, fo[0], fc[0], fm0 * opv(lP2, fo[0], fc[0]));
   __syncthreads();
-  // Pass 5: partials of the new basis at the cells.
+  // Pass 5: partials of the new basis at the cells. Products of two floats
+  // are exact in FP64, so each fma equals the product followed by the sum.
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     if (in[m]) {
-      const int j = j0 + TY * m + lj;
-      const int row = TY * m + lj + 4;
-      const int o = row * HQ + li + 4;
-      const float cv = op_at(lMA, row, li + 4, j, i);
-      const float ev = op_at(lZ2, row, li + 4, j, i);
+      const int o = o0 + m * TY * HQ;
+      const float cv = opv(lMA, o, cc[m]);
+      const float ev = opv(lZ2, o, cc[m]);
       const double p2 = lP2[o], z2 = lZ2[o], r2 = r2o[m], a2 = a2o[m];
       const double ma2 = mo[m] * a2o[m], c2 = cv, mc2 = mo[m] * cv, e2 = ev, me2 = mo[m] * ev;
-      acc[0] += p2 * a2;     // W01
-      acc[1] += p2 * c2;     // W02
-      acc[2] += p2 * e2;     // W04
-      acc[3] += a2 * ma2;    // W11
-      acc[4] += a2 * mc2;    // W12
-      acc[5] += a2 * z2;     // W13
-      acc[6] += a2 * me2;    // W14
-      acc[7] += c2 * mc2;    // W22
-      acc[8] += c2 * z2;     // W23
-      acc[9] += c2 * me2;    // W24
-      acc[10] += r2 * z2;    // W33
-      acc[11] += z2 * e2;    // W34
-      acc[12] += e2 * me2;   // W44
-      acc[13] += r2 * r2;    // |r|^2
+      acc[0] = __fma_rn(p2, a2, acc[0]);     // W01
+      acc[1] = __fma_rn(p2, c2, acc[1]);     // W02
+      acc[2] = __fma_rn(p2, e2, acc[2]);     // W04
+      acc[3] = __fma_rn(a2, ma2, acc[3]);    // W11
+      acc[4] = __fma_rn(a2, mc2, acc[4]);    // W12
+      acc[5] = __fma_rn(a2, z2, acc[5]);     // W13
+      acc[6] = __fma_rn(a2, me2, acc[6]);    // W14
+      acc[7] = __fma_rn(c2, mc2, acc[7]);    // W22
+      acc[8] = __fma_rn(c2, z2, acc[8]);     // W23
+      acc[9] = __fma_rn(c2, me2, acc[9]);    // W24
+      acc[10] = __fma_rn(r2, z2, acc[10]);   // W33
+      acc[11] = __fma_rn(z2, e2, acc[11]);   // W34
+      acc[12] = __fma_rn(e2, me2, acc[12]);  // W44
+      acc[13] = __fma_rn(r2, r2, acc[13]);   // |r|^2
     }
+  }
+}
+
+// Workgroup partials of the two-iteration round (KCA values per thread): a
+// transposed wave reduction. Each exchange pairs a lane with one whose index
+// differs in one more bit (lane distance 32 and 16 through the LDS crossbar,
+// then the DPP row mirror (index xor 15) and half-row mirror (xor 7)); the
+// lane keeps half of the values it carries and adds the partner's copy of
+// that half, so 16 values become 8, 4, 2, 1, and the quad sums (xor 2, xor 1)
+// finish. 17 FP64 additions per lane instead of 6 per value; lane 4q ends with
+// value q. Then thread k < KCA folds the NWAVE wave totals in wave order.
+template <int CTRL>
+__device__ __attribute__((always_inline)) inline double dpp_full(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+__device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], double* dst) {
+  static_assert(KCA <= 16 && NP >= 16, "16 slots carried through the exchange");
+  const int tid = tid_flat();
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int k = KCA; k < 16; ++k) v[k] = 0.0;
+  double a[8], b[4], c[2];
+  {
+    const bool hi = lane & 32;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double keep = hi ? v[8 + k] : v[k];
+      const double give = hi ? v[k] : v[8 + k];
+      a[k] = keep + __shfl_xor(give, 32);
+    }
+  }
+  {
+    const bool hi = lane & 16;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double keep = hi ? a[4 + k] : a[k];
+      const double give = hi ? a[k] : a[4 + k];
+      b[k] = keep + __shfl_xor(give, 16);
+    }
+  }
+  {
+    const bool hi = lane & 8;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double keep = hi ? b[2 + k] : b[k];
+      const double give = hi ? b[k] : b[2 + k];
+      c[k] = keep + dpp_full<0x140>(give);  // row_mirror
+    }
+  }
+  double d;
+  {
+    const bool hi = lane & 4;
+    const double keep = hi ? c[1] : c[0];
+    const double give = hi ? c[0] : c[1];
+    d = keep + dpp_full<0x141>(give);  // row_half_mirror
+  }
+  d = d + dpp_full<0x4E>(d);  // quad_perm [2,3,0,1]
+  d = d + dpp_full<0xB1>(d);  // quad_perm [1,0,3,2]
+  const int q = lane >> 2;
+  if ((lane & 3) == 0 && q < KCA) sm[wave][q] = d;
+  __syncthreads();
+  if (tid < KCA) {
+    double t = sm[0][tid];
+#pragma unroll
+    for (int w = 1; w < NWAVE; ++w) t += sm[w][tid];
+    dst[tid] = t;
   }
 }
 
@@ -1632,7 +1727,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
       if (r > r0) __syncthreads();  // the previous region's LDS reads are done
       region_ca<M>(C, A, r, acc, lds);
     }
-    block_partials_k<KCA, 0u>(acc, sm, part_out);
+    block_partials_ca(acc, sm, part_out);
   } else {
     round_cold<float>(phase, Cg, A, b, sm, part_out);
   }
