@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the float64 comparison run")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 GPU form")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tiles instead of timing the candidates first")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a plain 256 MiB device copy (10x) to calibrate PMC byte counters")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
@@ -90,11 +92,19 @@ REFINE_BYTES = 44
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_round_c2_1024_pmc_traffic.md")
 
 
-def measured_traffic(n, precision):
+# The tiling the traffic profile was taken with (the library's default at
+# 1024x1024: regions of 4 tiles, one per workgroup, two PCG iterations a round).
+PROFILED_TILE = (4, 256, 2)
+
+
+def measured_traffic(n, precision, tile=None):
     """HBM bytes per k_round launch measured with rocprofv3 FETCH_SIZE and
     WRITE_SIZE (calibrated on a plain copy) for the 1024x1024 mixed-precision
-    workload; None for other workloads or when the summary is absent."""
+    workload with the profiled tiling; None for other workloads or tilings,
+    or when the summary is absent."""
     if n != 1024 or precision != "mixed" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    if tile is not None and (tile.tile_rows, tile.max_workgroups, tile.pcg_per_round) != PROFILED_TILE:
         return None
     for line in open(TRAFFIC_PROFILE):
         if line.startswith("traffic_bytes_per_launch:"):
@@ -297,15 +307,38 @@ def plain_fp64_baseline(n, iters_per_step):
                       f"{b * 1e3:.3f} ms per iteration, extrapolated to {iters_per_step:.0f} iterations/step"}
 
 
+def tune_tile(args, nx, ny):
+    """The autotuner's tile for one cavity (timed on the device before the
+    measured run; None with --no-autotune)."""
+    if args.no_autotune:
+        return None
+    from pyflow import autotune
+    t0 = time.perf_counter()
+    tile = autotune.tune(nx, ny, args.precision, log=lambda m: log(args, m))
+    log(args, f"[bench] autotune {nx}x{ny}: tile_rows={tile.tile_rows} max_workgroups={tile.max_workgroups} "
+              f"pcg_per_round={tile.pcg_per_round} ({tile.us_per_iteration:.2f} us/iteration, "
+              f"{time.perf_counter() - t0:.1f} s)")
+    return tile
+
+
+def tile_config(tile):
+    if tile is None:
+        return "library default"
+    return {"tile_rows": tile.tile_rows, "max_workgroups": tile.max_workgroups,
+            "pcg_per_round": tile.pcg_per_round, "tuned_us_per_iteration": round(tile.us_per_iteration, 3)}
+
+
 def run_cavity(args):
     import numpy as np
     import torch
+    from pyflow import autotune
     from pyflow._device import DeviceBatch
     n = args.n
     N = n * n
     s = 4 if args.precision == "mixed" else 8
     params = c2_params(n, args.inner_rtol)
-    batch = DeviceBatch([params], args.precision)
+    tile = tune_tile(args, n, n)
+    batch = DeviceBatch([params if tile is None else autotune.apply(params, tile)], args.precision)
     if args.warmup > 0:
         log(args, f"[bench] warmup {args.warmup} steps at {n}x{n}")
         batch.run(args.warmup)
@@ -384,7 +417,8 @@ def run_cavity(args):
         "data": "synthetic: lid-driven cavity from rest (zero u, v, p)",
         "config": {"workload": f"lid-driven cavity {n}x{n}, Re=1000, U=1, QUICK, {args.steps} projection steps, "
                                "PCG tol 1e-6, lin_maxiter 10000, dx=dy=1",
-                   "grid": [n, n], "precision": args.precision, "inner_rtol": args.inner_rtol},
+                   "grid": [n, n], "precision": args.precision, "inner_rtol": args.inner_rtol,
+                   "tile": tile_config(tile)},
         "roofline": {
             "bound": "hbm",
             "kernel": "pf::k_round (one launch = one phase round of the cavity; PCG rounds dominate)",
@@ -392,7 +426,7 @@ def run_cavity(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n, args.precision),
+            "traffic": measured_traffic(n, args.precision, tile),
             "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "alg_bytes_per_launch": int(alg_bytes / max(rounds, 1)),
             "launches": rounds,
@@ -460,6 +494,7 @@ def run_poisson(args):
     import numpy as np
     import torch
     from pyflow import _hip as H
+    from pyflow import autotune
     from pyflow._device import DeviceBatch, cavity_params
     n = 4096 if args.n == 1024 else args.n
     N = n * n
@@ -469,6 +504,9 @@ def run_poisson(args):
     rhs[0, 0] = 0.0
     params = cavity_params(n, n, dx=1.0, dy=1.0, lin_tol=tol, lin_maxiter=1_000_000, jacobi=True,
                            op=H.OP_NEG_LAP_PINNED, mode=H.MODE_SOLVE, inner_rtol=args.inner_rtol)
+    tile = tune_tile(args, n, n)
+    if tile is not None:
+        autotune.apply(params, tile)
     batch = DeviceBatch([params], "mixed")
     batch.set_rhs(0, rhs)
     if args.warmup > 0:
@@ -504,7 +542,7 @@ def run_poisson(args):
         "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic: standard-normal right-hand side, seed 0",
         "config": {"workload": f"pinned-Neumann Poisson {n}x{n}, Jacobi-PCG, tol 1e-10, x0 = 0", "grid": [n, n],
-                   "inner_rtol": args.inner_rtol},
+                   "inner_rtol": args.inner_rtol, "tile": tile_config(tile)},
         "roofline": gpu_roofline("pf::k_round (one launch = one PCG iteration of the solve)", alg_bytes, gpu_ms,
                                  rounds),
         "pcg": {"iterations": its, "refinement_passes": outer, "rel_residual": rel,
@@ -529,22 +567,30 @@ def batch_params(count, seed, inner_rtol):
     return params, ns
 
 
-# The 256-cavity batch: at most 16 workgroups per cavity (several regions per
-# workgroup on the larger cavities) -- a round then holds ~1500 instead of ~6000
-# workgroups, each paying its state round trip and tail once (+10% measured).
+# The 256-cavity batch: the autotuner picks the per-cavity workgroup cap for
+# the whole batch (several regions per workgroup on the larger cavities, so a
+# round holds fewer workgroups, each paying its state round trip and tail once).
+# Without it: at most 16 workgroups per cavity (+10% over 256, measured).
 # Served requests keep the default (256): few cavities run at a time there, and
 # a large one capped at 16 workgroups would leave most CUs idle.
-MANY_CAVITY_MAXWG = "16"
+MANY_CAVITY_MAXWG = 16
 
 
 def run_batch(args):
-    os.environ.setdefault("PYFLOW_MAXWG", MANY_CAVITY_MAXWG)
     import numpy as np
     import torch
+    from pyflow import autotune
     from pyflow._device import DeviceBatch
     steps = args.steps
     params, ns = batch_params(256, 0, args.inner_rtol)
     cells = int(np.sum(ns.astype(np.int64) ** 2))
+    cap, tuned = MANY_CAVITY_MAXWG, None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        cap, tuned = autotune.tune_batch(params, args.precision, log=lambda m: log(args, m))
+        log(args, f"[bench] autotune batch: max_workgroups={cap} ({time.perf_counter() - t0:.1f} s)")
+    for p in params:
+        p.max_workgroups = cap
     batch = DeviceBatch(params, args.precision)
     if args.warmup > 0:
         log(args, f"[bench] warmup {args.warmup} steps, 256 cavities")
@@ -581,7 +627,9 @@ def run_batch(args):
                 "seed 0",
         "config": {"workload": f"256 cavities, QUICK, tol 1e-6, lin_maxiter 10000, {steps} steps", "cells": cells,
                    "precision": args.precision, "inner_rtol": args.inner_rtol,
-                   "max_workgroups_per_cavity": int(os.environ["PYFLOW_MAXWG"])},
+                   "max_workgroups_per_cavity": cap,
+                   "autotune_ps_per_cell_iteration": None if tuned is None else
+                   {str(k): round(v * 1e6, 4) for k, v in tuned.items()}},
         "roofline": gpu_roofline("pf::k_round<float, U, false> (all cavities' regions in one launch)", alg_bytes,
                                  gpu_ms, rounds),
         "pcg": {"iterations_total": int(its.sum()), "mean_per_cavity_step": round(float(its.mean()) / steps, 1)},

@@ -82,6 +82,13 @@ typedef struct pf_cavity_params {
   double lin_tol;
   double inv_diag;         /* 1/(2/dx**2 + 2/dy**2): interior Jacobi weight */
   double inner_rtol;       /* MIXED only: relative tolerance of each FP32 inner PCG pass */
+  /* Tiling and schedule; 0 = the library's default. pyflow.autotune times
+   * the candidates on the device and picks them. Changing them changes the
+   * order of the FP64 partial sums, not the arithmetic of any cell. */
+  int32_t tile_rows;       /* region height in 16-row tiles: 1, 2, 4 or 8 (regions are 64 cells wide) */
+  int32_t max_workgroups;  /* cap on the cavity's workgroups (regions are grouped to fit) */
+  int32_t pcg_per_round;   /* MIXED, FP32 iterate: PCG iterations per round, 1 or 2 (s-step, s = 2) */
+  int32_t reserved;        /* 0 */
 } pf_cavity_params;
 
 /* Per-cavity results of the last completed step / projection / solve. */
@@ -197,6 +204,16 @@ int pf_batch_rounds(pf_batch* b, int64_t* rounds);
  * (host-side polling gaps excluded). Enabling resets both counters. */
 int pf_batch_set_timing(pf_batch* b, int enable);
 int pf_batch_timing(pf_batch* b, double* gpu_ms, int64_t* rounds);
+
+/* Rounds are queued as replays of an instantiated hipGraph: a chain of
+ * `rounds` kernel nodes (rounded down to an even count, at most 1024; 0
+ * queues every round with its own launch). Replays and direct launches run
+ * the same kernels with the same arguments, so results are bitwise equal.
+ * The graphs are rebuilt after any change of the batch's cavities. Default:
+ * PYFLOW_GRAPH if set, else 16. The reference has no counterpart (its step
+ * runs on the host: solvers/solver.py:15); this serves the batched and
+ * served paths (drivers/simulation_driver.py:58-110 per cavity). */
+int pf_batch_set_graph(pf_batch* b, int rounds);
 
 /* Standalone float64 operator on device arrays of shape (ny, nx), dense rows.
  * Unused inputs/outputs may be NULL. */

@@ -85,6 +85,8 @@ struct pf_batch {
   bool any_xd = false;   // some cavity keeps its PCG iterate in FP64
   bool multi = false;    // some cavity has several regions per workgroup
   bool tables_dirty = true;
+  int graph_len = 0;                      // rounds per captured graph (even; 0: direct launches)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by parity of the first round
   int64_t rounds = 0;
   int64_t predicted_rounds = 0;
   int64_t round_guard = 0;
@@ -114,6 +116,10 @@ static int check_params(const pf_cavity_params* p) {
   if (p->scheme < 0 || p->scheme > 2) return fail(PF_EINVAL, "bad scheme");
   if (p->op < 0 || p->op > 1) return fail(PF_EINVAL, "bad op");
   if (p->mode < 0 || p->mode > 2) return fail(PF_EINVAL, "bad mode");
+  if (p->tile_rows != 0 && p->tile_rows != 1 && p->tile_rows != 2 && p->tile_rows != 4 && p->tile_rows != 8)
+    return fail(PF_EINVAL, "tile_rows must be 0, 1, 2, 4 or 8");
+  if (p->max_workgroups < 0) return fail(PF_EINVAL, "max_workgroups must be >= 0");
+  if (p->pcg_per_round < 0 || p->pcg_per_round > 2) return fail(PF_EINVAL, "pcg_per_round must be 0, 1 or 2");
   return PF_OK;
 }
 
@@ -138,13 +144,15 @@ static int env_switch(const char* name, int dflt) {
 // rpw consecutive regions per workgroup so that a cavity launches at most
 // PYFLOW_MAXWG (256: one per CU) workgroups: a workgroup then pays its
 // state/partials round trip and tail once per round, not once per region.
-static void blocking_of(int tiles_x, int tiles_y, int* nblk, int* tpb, int* nreg, int* rpw) {
+// A cavity's tile_rows / max_workgroups override both defaults.
+static void blocking_of(const pf_cavity_params* p, int tiles_x, int tiles_y, int* nblk, int* tpb, int* nreg,
+                        int* rpw) {
   int m = 1;
-  const int target = env_int("PYFLOW_TPB", 4);
+  const int target = p->tile_rows > 0 ? p->tile_rows : env_int("PYFLOW_TPB", 4);
   while (m * 2 <= target && m < 8) m *= 2;
   while (m > 1 && m / 2 >= tiles_y) m /= 2;  // no more rows than the cavity has
   const int regions = tiles_x * ((tiles_y + m - 1) / m);
-  const int maxwg = std::max(1, env_int("PYFLOW_MAXWG", 256));
+  const int maxwg = std::max(1, p->max_workgroups > 0 ? p->max_workgroups : env_int("PYFLOW_MAXWG", 256));
   int k = 1;
   while ((regions + k - 1) / k > maxwg) k *= 2;
   *tpb = m;
@@ -158,7 +166,7 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   int tx, ty, nb, tpb;
   tiles_of(p->nx, p->ny, &tx, &ty);
   int nreg, rpw;
-  blocking_of(tx, ty, &nb, &tpb, &nreg, &rpw);
+  blocking_of(p, tx, ty, &nb, &tpb, &nreg, &rpw);
   size_t s = 0;
   s += 10 * align256(n * esize(prec));  // u v p us vs r d0 d1 x r1
   s += align256(n * sizeof(double));                           // b
@@ -168,9 +176,11 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
 }
 
 // MIXED with an FP32 iterate: two PCG iterations per round (regions of up to
-// 4 tiles; their 4-cell frames fit in LDS). PYFLOW_CA=0 keeps one.
-static void choose_ca(PfCav& c, int prec) {
-  c.ca = (prec == PF_PREC_MIXED && !c.xd && c.tpb <= 4 && env_switch("PYFLOW_CA", 1) != 0) ? 1 : 0;
+// 4 tiles; their 4-cell frames fit in LDS). pcg_per_round = 1, or PYFLOW_CA=0
+// when the cavity leaves it at 0, keeps one.
+static void choose_ca(PfCav& c, int prec, int pcg_per_round) {
+  const int two = pcg_per_round > 0 ? (pcg_per_round == 2) : (env_switch("PYFLOW_CA", 1) != 0);
+  c.ca = (prec == PF_PREC_MIXED && !c.xd && c.tpb <= 4 && two) ? 1 : 0;
 }
 
 static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
@@ -180,7 +190,7 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   c.ntiles = tiles_of(p->nx, p->ny, &tx, &ty);
   c.tiles_x = tx;
   c.tiles_y = ty;
-  blocking_of(tx, ty, &c.nblk, &c.tpb, &c.nreg, &c.rpw);
+  blocking_of(p, tx, ty, &c.nblk, &c.tpb, &c.nreg, &c.rpw);
   c.scheme = p->scheme;
   c.lid_on = p->lid_enabled ? 1 : 0;
   c.jacobi = p->jacobi ? 1 : 0;
@@ -205,11 +215,14 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   const int xd_env = env_switch("PYFLOW_XD", -1);
   const int nmax = std::max(p->nx, p->ny);
   c.xd = (prec == PF_PREC_MIXED) ? (xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0)) : 0;
-  choose_ca(c, prec);
+  choose_ca(c, prec, p->pcg_per_round);
 }
+
+static void drop_graphs(pf_batch* b);
 
 static int sync_tables(pf_batch* b) {
   if (!b->tables_dirty) return PF_OK;
+  drop_graphs(b);
   std::vector<int> tiles;  // workgroup -> cavity
   tiles.reserve(b->max_tiles);
   for (int s = 0; s < b->max_cav; ++s) {
@@ -279,35 +292,43 @@ static int check_slot(pf_batch* b, int slot) {
   return PF_OK;
 }
 
+// The kernel instance a round of this batch runs, with its launch shape.
+struct RoundLaunch {
+  const void* fn = nullptr;
+  dim3 grid, blk;
+  size_t lds = 0;
+};
+
 template <class T, int U, bool MULTI>
-static void launch_round_m(pf_batch* b, const KArgs& K) {
-  dim3 blk(TX, TY);
-  const size_t lds = (sizeof(T) == sizeof(float)) ? (size_t)b->lds_floats * sizeof(float) : 0;
-  const dim3 grid(b->total_blocks);
-  if (sizeof(T) == sizeof(float) && b->any_xd) {
-    // FP64-iterate cavities are large (a round takes 100+ us): the descriptor
-    // is read from the table (the by-value copy would only add register
-    // pressure to the bigger kernel).
-    k_round<T, U, false, true, MULTI><<<grid, blk, lds, b->stream>>>(K);
-  } else {
-    if (b->single_slot >= 0)
-      k_round<T, U, true, false, MULTI><<<grid, blk, lds, b->stream>>>(K);
-    else
-      k_round<T, U, false, false, MULTI><<<grid, blk, lds, b->stream>>>(K);
-  }
+static const void* round_fn_m(const pf_batch* b) {
+  // FP64-iterate cavities are large (a round takes 100+ us): their descriptor
+  // is read from the table (the by-value copy would only add register
+  // pressure to the bigger kernel).
+  if (sizeof(T) == sizeof(float) && b->any_xd) return (const void*)k_round<T, U, false, true, MULTI>;
+  if (b->single_slot >= 0) return (const void*)k_round<T, U, true, false, MULTI>;
+  return (const void*)k_round<T, U, false, false, MULTI>;
 }
 
 template <class T, int U>
-static void launch_round_u(pf_batch* b, const KArgs& K) {
-  if (b->multi)
-    launch_round_m<T, U, true>(b, K);
-  else
-    launch_round_m<T, U, false>(b, K);
+static const void* round_fn_u(const pf_batch* b) {
+  return b->multi ? round_fn_m<T, U, true>(b) : round_fn_m<T, U, false>(b);
 }
 
-template <class T>
-static void launch_round(pf_batch* b) {
-  if (b->total_blocks <= 0) return;
+static RoundLaunch round_launch(const pf_batch* b) {
+  RoundLaunch L;
+  L.blk = dim3(TX, TY);
+  L.grid = dim3(b->total_blocks);
+  if (b->prec == PF_PREC_MIXED) {
+    L.fn = round_fn_u<float, 2>(b);  // U only shapes the FP64 P/Q loops
+    L.lds = (size_t)b->lds_floats * sizeof(float);
+  } else {
+    L.fn = b->unroll >= 4 ? round_fn_u<double, 4>(b) : b->unroll == 2 ? round_fn_u<double, 2>(b)
+                                                                      : round_fn_u<double, 1>(b);
+  }
+  return L;
+}
+
+static KArgs round_args(const pf_batch* b, int parity) {
   KArgs K;
   std::memset(&K, 0, sizeof(K));
   K.cavs = b->dcav;
@@ -315,30 +336,85 @@ static void launch_round(pf_batch* b) {
   K.blk_cav = b->dtile;
   K.active = b->dactive;
   K.max_cav = b->max_cav;
-  K.parity = (int)(b->round & 1);
+  K.parity = parity;
   K.slot0 = b->single_slot;
   if (b->single_slot >= 0) K.cav0 = b->hcav[b->single_slot];
-  if constexpr (sizeof(T) == sizeof(float)) {
-    launch_round_u<T, 2>(b, K);  // U only shapes the FP64 P/Q loops
-  } else {
-    if (b->unroll >= 4)
-      launch_round_u<T, 4>(b, K);
-    else if (b->unroll == 2)
-      launch_round_u<T, 2>(b, K);
-    else
-      launch_round_u<T, 1>(b, K);
-  }
-  b->round += 1;
+  return K;
 }
 
-static void launch_rounds(pf_batch* b, int64_t n) {
+static hipError_t launch_rounds_direct(pf_batch* b, int64_t n) {
+  if (b->total_blocks <= 0 || n <= 0) return hipSuccess;
+  const RoundLaunch L = round_launch(b);
   for (int64_t r = 0; r < n; ++r) {
-    if (b->prec == PF_PREC_MIXED)
-      launch_round<float>(b);
-    else
-      launch_round<double>(b);
+    KArgs K = round_args(b, (int)(b->round & 1));
+    void* args[] = {&K};
+    const hipError_t e = hipLaunchKernel(L.fn, L.grid, L.blk, args, L.lds, b->stream);
+    if (e != hipSuccess) return e;
+    b->round += 1;
   }
+  return hipSuccess;
+}
+
+static void drop_graphs(pf_batch* b) {
+  for (int par = 0; par < 2; ++par)
+    if (b->gexec[par]) {
+      (void)hipGraphExecDestroy(b->gexec[par]);
+      b->gexec[par] = nullptr;
+    }
+}
+
+// A chain of graph_len rounds whose first round has parity `par`: kernel
+// nodes only (each round depends on the one before), so a replay queues
+// graph_len rounds with one host call. The kernel arguments are fixed when
+// the graph is built; any change of the batch's tables drops the graphs.
+static int build_graph(pf_batch* b, int par) {
+  const RoundLaunch L = round_launch(b);
+  hipGraph_t g = nullptr;
+  HIPCHK(hipGraphCreate(&g, 0));
+  hipGraphNode_t prev = nullptr;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < b->graph_len && e == hipSuccess; ++k) {
+    KArgs K = round_args(b, par ^ (k & 1));
+    void* args[] = {&K};
+    hipKernelNodeParams np;
+    std::memset(&np, 0, sizeof(np));
+    np.func = const_cast<void*>(L.fn);
+    np.gridDim = L.grid;
+    np.blockDim = L.blk;
+    np.sharedMemBytes = (unsigned)L.lds;
+    np.kernelParams = args;
+    np.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &np);
+    prev = node;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(PF_EHIP, std::string("graph build failed: ") + hipGetErrorString(e));
+  b->gexec[par] = ex;
+  return PF_OK;
+}
+
+// Queues n rounds: whole graph replays where graphs are on, the rest (and
+// everything when they are off) as direct launches. Either way the same
+// kernels run with the same arguments in the same order.
+static int launch_rounds(pf_batch* b, int64_t n) {
+  if (b->total_blocks > 0 && b->graph_len > 0 && n >= b->graph_len) {
+    const int par = (int)(b->round & 1);
+    if (!b->gexec[par]) {
+      const int rc = build_graph(b, par);
+      if (rc) return rc;
+    }
+    const int64_t reps = n / b->graph_len;
+    for (int64_t k = 0; k < reps; ++k) HIPCHK(hipGraphLaunch(b->gexec[par], b->stream));
+    b->round += reps * b->graph_len;  // graph_len is even: the parity is unchanged
+    b->rounds += reps * b->graph_len;
+    n -= reps * b->graph_len;
+  }
+  HIPCHK(launch_rounds_direct(b, n));
   b->rounds += n;
+  return PF_OK;
 }
 
 static void fill_diag(const pf_batch* b, int slot, pf_step_diag* d) {
@@ -433,6 +509,9 @@ int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* pa
 // The float32 PCG regions stage up to 5 x 72 x 72 floats (101 KiB; the
 // two-iteration round on 4 tiles) in LDS, above the 64 KiB a launch may
 // request without opting in.
+// Rounds per graph: even (a replay then keeps the parity sequence), 0 = off.
+static int graph_len_of(int rounds) { return rounds <= 0 ? 0 : std::min(1024, std::max(2, rounds & ~1)); }
+
 static int allow_region_lds() {
   const int bytes = std::max(region_lds_floats(8), ca_lds_floats(4)) * (int)sizeof(float);
   const void* fns[6] = {
@@ -457,6 +536,7 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->stream = (hipStream_t)stream;
   b->hcav.assign(max_cav, PfCav{});
   b->unroll = env_int("PYFLOW_UNROLL", 2);
+  b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 16));
   b->used.assign(max_cav, 0);
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dcav, sizeof(PfCav) * max_cav);
@@ -588,6 +668,7 @@ int pf_batch_create(int precision, int ncav, const pf_cavity_params* params, voi
 int pf_batch_destroy(pf_batch* b) {
   if (!b) return PF_OK;
   (void)hipStreamSynchronize(b->stream);
+  drop_graphs(b);
   if (b->dcav) (void)hipFree(b->dcav);
   if (b->dscal) (void)hipFree(b->dscal);
   if (b->dtile) (void)hipFree(b->dtile);
@@ -616,7 +697,7 @@ int pf_batch_set_params(pf_batch* b, int slot, const pf_cavity_params* params) {
   c.tpb = tpb;
   c.nreg = nreg;
   c.rpw = rpw;
-  choose_ca(c, b->prec);
+  choose_ca(c, b->prec, params->pcg_per_round);
   b->tables_dirty = true;
   return PF_OK;
 }
@@ -764,7 +845,8 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
 
 static int chunk(pf_batch* b, int64_t n) {
   if (b->timing) HIPCHK(hipEventRecord(b->ev_t0, b->stream));
-  launch_rounds(b, n);
+  const int rc = launch_rounds(b, n);
+  if (rc) return rc;
   HIPCHK(hipGetLastError());
   if (b->timing) {
     HIPCHK(hipEventRecord(b->ev_t1, b->stream));
@@ -928,6 +1010,14 @@ int pf_batch_centerline(pf_batch* b, int slot, double* out) {
   HIPCHK(hipStreamSynchronize(b->stream));
   for (int j = 0; j < c.ny; ++j)
     out[j] = (es == sizeof(float)) ? (double)((const float*)tmp.data())[j] : ((const double*)tmp.data())[j];
+  return PF_OK;
+}
+
+int pf_batch_set_graph(pf_batch* b, int rounds) {
+  if (!b || rounds < 0) return fail(PF_EINVAL, "bad arguments");
+  if (b->chunk_pending) return fail(PF_ESTATE, "batch is running");
+  drop_graphs(b);
+  b->graph_len = graph_len_of(rounds);
   return PF_OK;
 }
 

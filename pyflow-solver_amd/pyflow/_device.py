@@ -240,6 +240,10 @@ class DeviceBatch:
         H.check(self.L.pf_batch_rounds(self.handle, C.byref(r)), "pf_batch_rounds")
         return int(r.value)
 
+    def set_graph(self, rounds: int) -> None:
+        """Rounds per replayed hipGraph (0: one launch per round)."""
+        H.check(self.L.pf_batch_set_graph(self.handle, int(rounds)), "pf_batch_set_graph")
+
     def set_timing(self, enable: bool = True) -> None:
         H.check(self.L.pf_batch_set_timing(self.handle, 1 if enable else 0), "pf_batch_set_timing")
 

@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "pf_batch_zero_fields", "pf_batch_set_meta", "pf_batch_set_rhs", "pf_batch_get_solution",
     "pf_batch_set_dt", "pf_batch_run", "pf_batch_launch", "pf_batch_poll", "pf_batch_diag",
     "pf_batch_centerline", "pf_batch_rounds", "pf_batch_set_timing", "pf_batch_timing", "pf_op_apply",
-    "pf_batch_start", "pf_batch_advance", "pf_batch_finished",
+    "pf_batch_start", "pf_batch_advance", "pf_batch_finished", "pf_batch_set_graph",
 )
 
 
@@ -54,6 +54,8 @@ class CavityParams(C.Structure):
         ("lid_velocity", C.c_double), ("dx", C.c_double), ("dy", C.c_double), ("nu", C.c_double),
         ("cfl_target", C.c_double), ("cfl_growth", C.c_double), ("lin_tol", C.c_double),
         ("inv_diag", C.c_double), ("inner_rtol", C.c_double),
+        ("tile_rows", C.c_int32), ("max_workgroups", C.c_int32), ("pcg_per_round", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -114,6 +116,7 @@ def _declare(L):
         "pf_batch_centerline": ([P, I, P], I),
         "pf_batch_rounds": ([P, C.POINTER(I64)], I),
         "pf_batch_set_timing": ([P, I], I),
+        "pf_batch_set_graph": ([P, I], I),
         "pf_batch_timing": ([P, C.POINTER(D), C.POINTER(I64)], I),
         "pf_op_apply": ([I, I, I, D, D, P, P, P, P, P, P], I),
         "pf_batch_start": ([P, I, I], I),

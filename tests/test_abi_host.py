@@ -193,3 +193,53 @@ def test_residual_manager_and_state_api():
     assert interior_view(st["u"]).shape == (4, 5)
     st2 = State(nx=2, ny=2, fields={"u": np.zeros((4, 4))})
     assert st2.get("u").shape == (4, 4) and st2.get("missing", 3) == 3
+
+
+def test_autotune_blocking_mirror_matches_library():
+    """The tuner's Python mirror of the library's blocking gives the same
+    workgroup count as the library: a MIXED workspace is 12 cell vectors plus
+    two parities x 16 FP64 partials per workgroup (256 B each)."""
+    import ctypes
+    import pyflow._hip as H
+    from pyflow import autotune as A
+    from pyflow._device import cavity_params
+    L = H.lib()
+    al = lambda x: (x + 255) // 256 * 256  # noqa: E731
+    rng = np.random.default_rng(3)
+    cases = [(1024, 1024, 4, 256), (4096, 4096, 4, 256), (130, 97, 8, 2), (33, 17, 1, 1), (2, 2, 8, 1024)]
+    cases += [(int(rng.integers(2, 3000)), int(rng.integers(2, 3000)), int(rng.choice([1, 2, 4, 8])),
+               int(rng.choice([1, 3, 32, 64, 256, 1024]))) for _ in range(40)]
+    for nx, ny, rows, cap in cases:
+        p = cavity_params(nx, ny, dx=1.0, dy=1.0)
+        p.tile_rows, p.max_workgroups = rows, cap
+        n = ctypes.c_size_t()
+        assert L.pf_batch_workspace_bytes(H.PREC_MIXED, 1, ctypes.byref(p), ctypes.byref(n)) == 0
+        tpb, regions, rpw, nblk = A.blocking(nx, ny, rows, cap)
+        assert nblk <= cap and rpw * nblk >= regions and tpb <= rows
+        assert n.value == 10 * al(nx * ny * 4) + 2 * al(nx * ny * 8) + al(2 * nblk * 16 * 8), (nx, ny, rows, cap)
+
+
+def test_tile_fields_validated():
+    import ctypes
+    import pyflow._hip as H
+    from pyflow._device import cavity_params
+    L = H.lib()
+    n = ctypes.c_size_t()
+    for field, bad in (("tile_rows", 3), ("tile_rows", 16), ("max_workgroups", -1), ("pcg_per_round", 3)):
+        p = cavity_params(64, 64, dx=1.0, dy=1.0)
+        setattr(p, field, bad)
+        assert L.pf_batch_workspace_bytes(1, 1, ctypes.byref(p), ctypes.byref(n)) == -1
+        assert field.encode() in L.pf_last_error()
+
+
+def test_autotune_candidates_are_distinct_and_valid():
+    from pyflow import autotune as A
+    for n, prec in ((1024, "mixed"), (4096, "mixed"), (96, "fp64"), (20, "mixed")):
+        cs = A.candidates(n, n, prec)
+        assert cs
+        keys = {(A.blocking(n, n, c.tile_rows, c.max_workgroups), c.pcg_per_round) for c in cs}
+        assert len(keys) == len(cs)
+        for c in cs:
+            assert c.pcg_per_round in (1, 2)
+            if c.pcg_per_round == 2:  # s-step only where the library runs it
+                assert prec == "mixed" and n < 1536 and A.blocking(n, n, c.tile_rows, c.max_workgroups)[0] <= 4

@@ -467,3 +467,35 @@ def test_several_regions_per_workgroup(precision, cap, monkeypatch):
     assert d.steps_done == 6 and not d.nan_detected
     assert float(np.max(np.abs(u - ref.fields["u"]))) <= cap
     assert float(np.max(np.abs(v - ref.fields["v"]))) <= cap
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_equal_direct_launches(precision):
+    """Rounds queued as hipGraph replays (16- and 6-round chains, the rest of
+    each chunk launched directly) give the same bits as one launch per round,
+    over several runs of the same batch with host work in between."""
+    from pyflow._device import DeviceBatch, cavity_params
+    sizes, res = [200, 96, 40], [1000.0, 300.0, 2000.0]
+    ps = [cavity_params(n, n, dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=re, lin_tol=1e-6,
+                        lin_maxiter=5000) for n, re in zip(sizes, res)]
+    outs = []
+    for glen in (0, 16, 6):
+        b = DeviceBatch(ps, precision)
+        b.set_graph(glen)
+        runs = []
+        for nsteps in (2, 1, 3):  # three runs: each replays the graphs of the same exec
+            d = b.run(nsteps)
+            fields = []
+            for k, n in enumerate(sizes):
+                u = np.zeros((n + 2, n + 2)); v = np.zeros_like(u); p = np.zeros_like(u)
+                b.download(k, u, v, p)
+                fields.append((u, v, p))
+            runs.append((fields, [(d[k].rp_iterations_run, d[k].continuity, d[k].dt) for k in range(len(sizes))]))
+        b.close()
+        outs.append(runs)
+    for glen_runs in outs[1:]:
+        for (f0, s0), (f1, s1) in zip(outs[0], glen_runs):
+            assert s0 == s1
+            for a, c in zip(f0, f1):
+                for x, y in zip(a, c):
+                    assert np.array_equal(x, y)
