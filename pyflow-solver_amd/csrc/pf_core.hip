@@ -83,6 +83,7 @@ struct pf_batch {
   int max_tpb = 1;       // largest region height (tiles) over the batch's cavities
   int lds_floats = 0;    // dynamic LDS of a float32 round: the largest PCG region's staging
   bool any_xd = false;   // some cavity keeps its PCG iterate in FP64
+  int xd_kinds = 0;      // rounds of those cavities: bit 0 one PCG iteration, bit 1 two
   bool multi = false;    // some cavity has several regions per workgroup
   bool tables_dirty = true;
   int graph_len = 0;                      // rounds per captured graph (even; 0: direct launches)
@@ -175,12 +176,14 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   return s;
 }
 
-// MIXED with an FP32 iterate: two PCG iterations per round (regions of up to
-// 4 tiles; their 4-cell frames fit in LDS). pcg_per_round = 1, or PYFLOW_CA=0
-// when the cavity leaves it at 0, keeps one.
+// MIXED: two PCG iterations per round (regions of up to 4 tiles; their
+// 4-cell frames fit in LDS), with an FP32 or an FP64 iterate. Default on;
+// PYFLOW_CA=0 (FP32 iterate) or PYFLOW_CA_XD=0 (FP64 iterate) turns it off,
+// and a cavity's pcg_per_round (1 or 2) overrides both.
 static void choose_ca(PfCav& c, int prec, int pcg_per_round) {
-  const int two = pcg_per_round > 0 ? (pcg_per_round == 2) : (env_switch("PYFLOW_CA", 1) != 0);
-  c.ca = (prec == PF_PREC_MIXED && !c.xd && c.tpb <= 4 && two) ? 1 : 0;
+  const int dflt = c.xd ? (env_switch("PYFLOW_CA_XD", 1) != 0) : (env_switch("PYFLOW_CA", 1) != 0);
+  const int two = pcg_per_round > 0 ? (pcg_per_round == 2) : dflt;
+  c.ca = (prec == PF_PREC_MIXED && c.tpb <= 4 && two) ? 1 : 0;
 }
 
 static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
@@ -243,6 +246,7 @@ static int sync_tables(pf_batch* b) {
   b->max_tpb = 1;
   b->lds_floats = 0;
   b->any_xd = false;
+  b->xd_kinds = 0;
   b->multi = false;
   for (int s = 0; s < b->max_cav; ++s)
     if (b->used[s]) {
@@ -250,6 +254,7 @@ static int sync_tables(pf_batch* b) {
       const int tpb = b->hcav[s].tpb;
       b->lds_floats = std::max(b->lds_floats, b->hcav[s].ca ? ca_lds_floats(tpb) : region_lds_floats(tpb));
       b->any_xd = b->any_xd || b->hcav[s].xd;
+      if (b->hcav[s].xd) b->xd_kinds |= b->hcav[s].ca ? 2 : 1;
       b->multi = b->multi || b->hcav[s].rpw > 1;
     }
   HIPCHK(hipMemcpyAsync(b->dcav, b->hcav.data(), sizeof(PfCav) * b->max_cav, hipMemcpyHostToDevice, b->stream));
@@ -304,9 +309,13 @@ static const void* round_fn_m(const pf_batch* b) {
   // FP64-iterate cavities are large (a round takes 100+ us): their descriptor
   // is read from the table (the by-value copy would only add register
   // pressure to the bigger kernel).
-  if (sizeof(T) == sizeof(float) && b->any_xd) return (const void*)k_round<T, U, false, true, MULTI>;
-  if (b->single_slot >= 0) return (const void*)k_round<T, U, true, false, MULTI>;
-  return (const void*)k_round<T, U, false, false, MULTI>;
+  if (sizeof(T) == sizeof(float) && b->any_xd) {
+    if (b->xd_kinds == 2) return (const void*)k_round<T, U, false, 2, MULTI>;
+    if (b->xd_kinds == 3) return (const void*)k_round<T, U, false, 3, MULTI>;
+    return (const void*)k_round<T, U, false, 1, MULTI>;
+  }
+  if (b->single_slot >= 0) return (const void*)k_round<T, U, true, 0, MULTI>;
+  return (const void*)k_round<T, U, false, 0, MULTI>;
 }
 
 template <class T, int U>
@@ -514,10 +523,12 @@ static int graph_len_of(int rounds) { return rounds <= 0 ? 0 : std::min(1024, st
 
 static int allow_region_lds() {
   const int bytes = std::max(region_lds_floats(8), ca_lds_floats(4)) * (int)sizeof(float);
-  const void* fns[6] = {
-      (const void*)k_round<float, 2, true, false, false>, (const void*)k_round<float, 2, false, false, false>,
-      (const void*)k_round<float, 2, false, true, false>, (const void*)k_round<float, 2, true, false, true>,
-      (const void*)k_round<float, 2, false, false, true>, (const void*)k_round<float, 2, false, true, true>};
+  const void* fns[10] = {
+      (const void*)k_round<float, 2, true, 0, false>,  (const void*)k_round<float, 2, false, 0, false>,
+      (const void*)k_round<float, 2, false, 1, false>, (const void*)k_round<float, 2, false, 2, false>,
+      (const void*)k_round<float, 2, false, 3, false>, (const void*)k_round<float, 2, true, 0, true>,
+      (const void*)k_round<float, 2, false, 0, true>,  (const void*)k_round<float, 2, false, 1, true>,
+      (const void*)k_round<float, 2, false, 2, true>,  (const void*)k_round<float, 2, false, 3, true>};
   for (const void* f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   return PF_OK;
 }

@@ -530,6 +530,7 @@ struct PhaseArgs {
   float cx, cy, betaf, alphaf, minvf;  // float32 path: 1/dx^2, 1/dy^2, CG scalars, Jacobi weight
   float ca0f, cb0f, ca1f, cb1f;        // two-iteration round: alpha_k, beta_k, alpha_k+1, beta_k+1
   int ca_init;                         // two-iteration round that only forms the starting basis
+  double ca0, ca1;                     // alpha_k, alpha_k+1 for an FP64 iterate
 };
 
 // ---------------------------------------------------------------------------
@@ -1149,7 +1150,9 @@ __device__ __attribute__((always_inline)) inline CaCell ca_steps(const PhaseArgs
   return o;
 }
 
-template <int M>
+// XD: the iterate is the FP64 x64 (x64 += alpha_k p_k + alpha_k+1 p_k+1 in
+// FP64), as in the one-iteration FP64-iterate round.
+template <int M, bool XD>
 __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, const PhaseArgs& A, int r, double* acc,
                                                                 float* __restrict__ lds) {
   const int nx = C.nx, ny = C.ny;
@@ -1216,17 +1219,24 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
   };
   // Pass 1: every global load of the thread, unconditional (out-of-range
   // lanes read cell 0 and discard it).
-  float ro[M], po[M], xo[M];
+  using XT = std::conditional_t<XD, double, float>;
+  float ro[M], po[M];
+  XT xo[M];
   bool in[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = j0 + TY * m + lj;
     in[m] = (i < nx) && (j < ny);
     const size_t k = in[m] ? (size_t)j * nx + i : 0;
-    const float rv = R[k], pv = Pold[k], xv = X[k];
+    const float rv = R[k], pv = Pold[k];
+    XT xv;
+    if constexpr (XD)
+      xv = C.x64[k];
+    else
+      xv = X[k];
     ro[m] = in[m] ? rv : 0.f;
     po[m] = (in[m] && !init) ? pv : 0.f;
-    xo[m] = in[m] ? xv : 0.f;
+    xo[m] = in[m] ? xv : XT(0);
   }
   float fr[NS], fp[NS];
   int fo[NS];
@@ -1302,7 +1312,10 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
       __builtin_nontemporal_store(v.p2, &Pn[k]);
       if (!init) {
         __builtin_nontemporal_store(v.r2, &Rn[k]);
-        __builtin_nontemporal_store((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1, &X[k]);
+        if constexpr (XD)
+          __builtin_nontemporal_store((xo[m] + A.ca0 * (double)po[m]) + A.ca1 * (double)v.p1, &C.x64[k]);
+        else
+          __builtin_nontemporal_store((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1, &X[k]);
       }
       r2o[m] = v.r2;
       put(lP2, o, cc[m], v.p2);
@@ -1636,7 +1649,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
 // partials are folded by one wave per slot: wave w sums slot w of every
 // workgroup (per lane: increasing workgroup index; then the DPP tree), so all
 // fourteen slots of the PCG round fold in parallel; thread 63 runs the tail.
-template <int M, bool MULTI>
+template <int M, bool MULTI, bool XD>
 __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c,
                                                                     int b, PfScal& st, double (*sm)[NP], double* tot,
                                                                     float* lds) {
@@ -1718,6 +1731,8 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   A.ca1f = (float)st.ca1;
   A.cb1f = (float)st.cb1;
   A.ca_init = (st.ca_inc == 0);
+  A.ca0 = st.ca0;
+  A.ca1 = st.ca1;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
   double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
   const int r0 = MULTI ? b * C.rpw : b;
@@ -1725,7 +1740,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   if (__builtin_expect(phase == PH_CG, 1)) {
     for (int r = r0; r < r1; ++r) {
       if (r > r0) __syncthreads();  // the previous region's LDS reads are done
-      region_ca<M>(C, A, r, acc, lds);
+      region_ca<M, XD>(C, A, r, acc, lds);
     }
     block_partials_ca(acc, sm, part_out);
   } else {
@@ -1739,10 +1754,11 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   }
 }
 
-// XDANY: some cavity of the batch keeps its PCG iterate in FP64; otherwise
-// only the FP32-iterate body is compiled, which keeps the kernel small.
+// XDK: which FP64-iterate bodies are compiled (bit 0: one PCG iteration per
+// round, bit 1: two); 0 when no cavity of the batch keeps its PCG iterate in
+// FP64, which keeps the kernel small.
 // MULTI: some cavity has several regions per workgroup.
-template <class T, int U, bool SINGLE, bool XDANY, bool MULTI>
+template <class T, int U, bool SINGLE, int XDK, bool MULTI>
 __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   __shared__ PfScal st;
   __shared__ double sm[NWAVE][NP];
@@ -1770,7 +1786,13 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   const int lg = g - C.blk0;
   const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
   if constexpr (sizeof(T) == sizeof(float)) {
-    if (XDANY && C.xd) {
+    if ((XDK & 2) && C.xd && C.ca) {
+      switch (C.tpb) {
+        case 1: round_body_ca<1, MULTI, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body_ca<2, MULTI, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body_ca<4, MULTI, true>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+      }
+    } else if ((XDK & 1) && C.xd) {
       switch (C.tpb) {
         case 1: round_body<T, U, 1, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
         case 2: round_body<T, U, 2, true, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
@@ -1779,9 +1801,9 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
       }
     } else if (C.ca) {
       switch (C.tpb) {
-        case 1: round_body_ca<1, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        case 2: round_body_ca<2, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
-        default: round_body_ca<4, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 1: round_body_ca<1, MULTI, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        case 2: round_body_ca<2, MULTI, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
+        default: round_body_ca<4, MULTI, false>(KA, C, Cg, c, b, st, sm, tot, lds_dyn); break;
       }
     } else {
       switch (C.tpb) {

@@ -7,8 +7,8 @@ The PCG round dominates every workload (about 98% of the launches of a
 * ``tile_rows``: region height in 16-row tiles (regions are 64 cells wide);
 * ``max_workgroups``: cap on the cavity's workgroups; regions are grouped so
   that each workgroup walks ``rpw`` consecutive regions;
-* ``pcg_per_round``: MIXED with an FP32 iterate, 1 or 2 PCG iterations per
-  round (s-step PCG, s = 2).
+* ``pcg_per_round``: MIXED, 1 or 2 PCG iterations per round (s-step PCG,
+  s = 2), with an FP32 or (on large grids) an FP64 iterate.
 
 They change where cells are computed and the order in which FP64 partial sums
 are folded, never the arithmetic of a cell. ``tune`` times each distinct
@@ -41,7 +41,6 @@ __all__ = ["Tile", "blocking", "candidates", "tune", "tune_batch", "apply", "cac
 TX, TY = 64, 16  # cells per tile (one 64-lane wave per tile row)
 _WG_CAPS = (32, 64, 128, 256, 512, 1024)
 _ROWS = (1, 2, 4, 8)
-_XD_MIN_N = 1536  # MIXED keeps an FP64 PCG iterate from this size on (no s-step there)
 
 
 @dataclass(frozen=True)
@@ -73,13 +72,12 @@ def candidates(nx: int, ny: int, precision: str | int = "mixed",
     """Distinct (tile_rows, max_workgroups, pcg_per_round) settings for a
     cavity: settings that give the library the same blocking are listed once."""
     prec = precision_code(precision)
-    xd = prec == H.PREC_MIXED and max(nx, ny) >= _XD_MIN_N
     seen, out = set(), []
     for r in rows:
         for cap in caps:
             tpb, _, rpw, nblk = blocking(nx, ny, r, cap)
             for s in (2, 1):
-                two = s == 2 and prec == H.PREC_MIXED and not xd and tpb <= 4
+                two = s == 2 and prec == H.PREC_MIXED and tpb <= 4
                 if s == 2 and not two:
                     continue
                 key = (tpb, rpw, nblk, two)

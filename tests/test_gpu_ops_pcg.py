@@ -133,11 +133,15 @@ def test_device_cg_mixed_reaches_fp64_tolerance(n):
 
 
 @pytest.mark.parametrize("n", [64, 200])
-def test_device_cg_mixed_fp64_iterate(n, monkeypatch):
-    """The FP64-iterate PCG (exact line search along each direction, used on
-    large grids) meets the FP64 stopping rule on the true residual."""
+@pytest.mark.parametrize("two", ["0", "1"])
+def test_device_cg_mixed_fp64_iterate(n, two, monkeypatch):
+    """The FP64-iterate PCG (used on large grids), one iteration per round
+    (exact line search along each direction) or two (s-step round with the
+    iterate accumulated in FP64), meets the FP64 stopping rule on the true
+    residual."""
     from pyflow.linear_solvers.interface import PinnedNegLaplacian, device_cg
     monkeypatch.setenv("PYFLOW_XD", "1")
+    monkeypatch.setenv("PYFLOW_CA_XD", two)
     b = _rhs(n, n, 11)
     op = PinnedNegLaplacian(n, n, 1.0, 1.0)
     mv = lambda z: O.neg_laplacian_pinned(z, n, n, 1.0, 1.0)  # noqa: E731
