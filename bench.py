@@ -89,24 +89,27 @@ def step_other_bytes(s):
 REFINE_BYTES = 44
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_round_c2_1024_pmc_traffic.md")
-
-
-# The tiling the traffic profile was taken with (the library's default at
-# 1024x1024: regions of 4 tiles, one per workgroup, two PCG iterations a round).
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summaries (calibrated on a plain copy) per
+# workload: the grid, and the tiling they were taken with (the library's
+# default, also the autotuner's pick: regions of 4 tiles, at most 256
+# workgroups, two PCG iterations a round).
+TRAFFIC_PROFILES = {
+    1024: os.path.join(ROOT, "profiles", "k_round_c2_1024_pmc_traffic.md"),
+    4096: os.path.join(ROOT, "profiles", "k_round_poisson_4096_pmc_traffic.md"),
+}
 PROFILED_TILE = (4, 256, 2)
 
 
 def measured_traffic(n, precision, tile=None):
-    """HBM bytes per k_round launch measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE (calibrated on a plain copy) for the 1024x1024 mixed-precision
-    workload with the profiled tiling; None for other workloads or tilings,
-    or when the summary is absent."""
-    if n != 1024 or precision != "mixed" or not os.path.exists(TRAFFIC_PROFILE):
+    """HBM bytes per k_round launch measured for this grid's mixed-precision
+    workload with the profiled tiling; None for other grids or tilings, or
+    when the summary is absent."""
+    path = TRAFFIC_PROFILES.get(n)
+    if path is None or precision != "mixed" or not os.path.exists(path):
         return None
     if tile is not None and (tile.tile_rows, tile.max_workgroups, tile.pcg_per_round) != PROFILED_TILE:
         return None
-    for line in open(TRAFFIC_PROFILE):
+    for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":", 1)[1])
     return None
@@ -543,8 +546,8 @@ def run_poisson(args):
         "dtype": "f32+f64", "data": "synthetic: standard-normal right-hand side, seed 0",
         "config": {"workload": f"pinned-Neumann Poisson {n}x{n}, Jacobi-PCG, tol 1e-10, x0 = 0", "grid": [n, n],
                    "inner_rtol": args.inner_rtol, "tile": tile_config(tile)},
-        "roofline": gpu_roofline("pf::k_round (one launch = one PCG iteration of the solve)", alg_bytes, gpu_ms,
-                                 rounds),
+        "roofline": gpu_roofline("pf::k_round (one launch = one PCG round of the solve: two iterations)",
+                                 alg_bytes, gpu_ms, rounds, measured_traffic(n, "mixed", tile)),
         "pcg": {"iterations": its, "refinement_passes": outer, "rel_residual": rel,
                 "hbm_frac": round(pcg_gbs / HBM_PEAK_GBS, 4), "solve_s": round(wall, 4)},
         "cpu_baseline": cpu,
