@@ -78,3 +78,34 @@ def test_autotune_returns_a_measured_candidate(tmp_path, monkeypatch):
     cap, res = A.tune_batch([_cavity(96, 96, 100.0, 1e-6, 100), _cavity(40, 72, 100.0, 1e-6, 100)], "mixed",
                             caps=(2, 16), warmup=4, rounds=8)
     assert cap in (2, 16) and all(np.isfinite(v) and v > 0 for v in res.values())
+
+
+def test_fp64_iterate_kinds_mixed_in_one_batch(monkeypatch):
+    """FP64-iterate cavities with one and with two PCG iterations per round in
+    one batch (the kernel instance that compiles both bodies) give the same
+    bits as each alone (instances with one body), and follow the FP64 oracle."""
+    from pyflow import autotune as A
+    from pyflow._device import DeviceBatch
+    monkeypatch.setenv("PYFLOW_XD", "1")
+    nx, ny, re, tol, maxiter = 130, 97, 400.0, 1e-10, 5000
+    specs = [(A.Tile(4, 1024, 2), 130, 97), (A.Tile(2, 3, 1), 96, 72), (A.Tile(1, 4, 2), 70, 40)]
+    ps = [A.apply(_cavity(x, y, re, tol, maxiter), t) for t, x, y in specs]
+    mix = DeviceBatch(ps, "mixed")
+    dm = mix.run(3)
+    for k, ((t, x, y), p) in enumerate(zip(specs, ps)):
+        alone = DeviceBatch([p], "mixed")
+        da = alone.run(3)[0]
+        ua = np.zeros((y + 2, x + 2)); alone.download(0, ua, None, None)
+        um = np.zeros((y + 2, x + 2)); mix.download(k, um, None, None)
+        alone.close()
+        assert np.array_equal(ua, um), k
+        assert da.rp_iterations_run == dm[k].rp_iterations_run, k
+        assert dm[k].rp_converged and not dm[k].nan_detected
+        if k == 0:
+            cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5,
+                                  cfl_growth=1.05, Re=re, lid_velocity=1.0, lin_tol=tol, lin_maxiter=maxiter)
+            ref = O.allocate_state(x, y)
+            for it in range(3):
+                O.step(cfg, ref, it)
+            assert float(np.max(np.abs(um - ref.fields["u"]))) <= 1e-5
+    mix.close()
