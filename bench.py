@@ -656,43 +656,38 @@ def serve_requests(count, seed, step_scale):
 
 
 def serve(reqs, arrivals, precision, inner_rtol, chunk_rounds=64, max_cav=64, log_fn=None):
-    """Continuous batching: admit arrived requests into the running batch,
-    advance it in chunks, retire finished cavities. Returns per-request
-    completion times (seconds from the start) and the makespan."""
+    """Continuous batching through pyflow.serving.CavityServer: requests are
+    submitted at their arrival times, admitted by HBM footprint into the
+    running batch, advanced in chunks and retired when done. Returns
+    per-request completion times (seconds from the start), the makespan and
+    the server's peak concurrency and device bytes."""
     import collections
-    from pyflow._device import DeviceBatch, cavity_params
+    from pyflow.serving import CavityRequest, CavityServer
     ns, res, steps = reqs
-    srv = DeviceBatch(None, precision, max_cav=max_cav, max_tiles=1 << 20)
+    srv = CavityServer(precision, max_cavities=max_cav, chunk_rounds=chunk_rounds, inner_rtol=inner_rtol)
     order = collections.deque(sorted(range(len(ns)), key=lambda k: arrivals[k]))
-    active, done = {}, {}
+    ticket_of, done = {}, {}
     t0 = time.perf_counter()
     last_log = t0
-    while order or active:
+    while order or not srv.idle():
         now = time.perf_counter() - t0
-        while order and arrivals[order[0]] <= now and len(active) < max_cav:
+        while order and arrivals[order[0]] <= now:
             k = order.popleft()
-            p = cavity_params(int(ns[k]), int(ns[k]), dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0,
-                              Re=float(res[k]), cfl_target=0.5, cfl_growth=1.05, lin_tol=1e-6, lin_maxiter=10000,
-                              jacobi=True, inner_rtol=inner_rtol)
-            slot = srv.add(p)
-            srv.start(slot, int(steps[k]))
-            active[slot] = k
-        if not active:
+            req = CavityRequest(int(ns[k]), int(ns[k]), int(steps[k]), Re=float(res[k]), lid_velocity=1.0,
+                                scheme="quick", lin_tol=1e-6, lin_maxiter=10000)
+            ticket_of[srv.submit(req, t0 + arrivals[k])] = k
+        if srv.idle():
             time.sleep(max(0.0, arrivals[order[0]] - (time.perf_counter() - t0)))
             continue
-        srv.advance(chunk_rounds)
-        while not srv.poll():
-            time.sleep(0.0001)
-        for slot in srv.finished():
-            k = active.pop(slot)
-            done[k] = time.perf_counter() - t0
-            srv.remove(slot)
+        for r in srv.pump():
+            done[ticket_of[r.ticket]] = r.t_done - t0
         if log_fn and time.perf_counter() - last_log > 10.0:
             last_log = time.perf_counter()
-            log_fn(f"[bench] serving: {len(done)} done, {len(active)} running, {len(order)} waiting")
+            log_fn(f"[bench] serving: {len(done)} done, {srv.running} running, {srv.waiting + len(order)} waiting")
     makespan = time.perf_counter() - t0
+    peak = (srv.peak_running, srv.peak_bytes)
     srv.close()
-    return done, makespan
+    return done, makespan, peak
 
 
 def run_serve(args):
@@ -706,12 +701,13 @@ def run_serve(args):
     m = min(8, count)
     log(args, f"[bench] capacity probe: {m} requests at once")
     cal = (ns[:m], res[:m], steps[:m])
-    _, span = serve(cal, np.zeros(m), args.precision, args.inner_rtol)
+    _, span, _ = serve(cal, np.zeros(m), args.precision, args.inner_rtol)
     capacity = float(np.sum(work[:m])) / span  # cell-steps/s
     lam = 0.7 * capacity / float(np.mean(work))  # requests/s at 70% of capacity
     arrivals = np.cumsum(np.random.default_rng(1).exponential(1.0 / lam, size=count))
     log(args, f"[bench] capacity {capacity / 1e6:.1f} Mcell-steps/s; arrival rate {lam:.3f}/s; {count} requests")
-    done, makespan = serve(reqs, arrivals, args.precision, args.inner_rtol, log_fn=lambda m_: log(args, m_))
+    done, makespan, (peak_running, peak_bytes) = serve(reqs, arrivals, args.precision, args.inner_rtol,
+                                                       log_fn=lambda m_: log(args, m_))
     lat = np.array([done[k] - arrivals[k] for k in range(count)])
     out = {
         "metric": "p50/p99 served-solve latency (continuous batching of lid-driven cavity requests)",
@@ -722,7 +718,9 @@ def run_serve(args):
         "data": ("synthetic: Poisson arrivals (seed 1) at 70% of the measured capacity; n ~ round(exp(U(ln 64, "
                  f"ln 2048))), Re log-uniform [100, 1e4], steps ~ U{{100..2000}} x {args.step_scale} (seed 0)"),
         "config": {"workload": f"{count} cavity requests, QUICK, tol 1e-6, lin_maxiter 10000", "requests": count,
-                   "arrival_rate_per_s": round(lam, 4), "precision": args.precision},
+                   "arrival_rate_per_s": round(lam, 4), "precision": args.precision,
+                   "admission": "HBM footprint, first fit in arrival order (pyflow.serving.CavityServer)",
+                   "peak_running": peak_running, "peak_device_bytes": int(peak_bytes)},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "mean": round(float(np.mean(lat)), 4), "max": round(float(np.max(lat)), 4)},
         "throughput_mcell_steps_s": round(float(np.sum(work)) / makespan / 1e6, 3),

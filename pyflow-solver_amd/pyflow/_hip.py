@@ -129,11 +129,24 @@ def _declare(L):
         fn.restype = res
 
 
+def _torch_runtime_first() -> None:
+    # On a machine with a GPU, torch initialises its HIP runtime before this
+    # library is loaded (the library registers its code objects when it is
+    # loaded). Loaded the other way round, the library's hipSetDevice found no
+    # device in that process.
+    try:
+        import torch
+    except ImportError:  # pragma: no cover - torch is a dependency
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     """Load (once) and return the ctypes handle of libpyflow_hip.so."""
     global _lib
     if _lib is not None:
         return _lib
+    _torch_runtime_first()
     with _lock:
         if _lib is None:
             path = library_path()

@@ -243,3 +243,24 @@ def test_autotune_candidates_are_distinct_and_valid():
             assert c.pcg_per_round in (1, 2)
             if c.pcg_per_round == 2:  # s-step only where the library runs it
                 assert prec == "mixed" and A.blocking(n, n, c.tile_rows, c.max_workgroups)[0] <= 4
+
+
+def test_serving_footprint_and_first_fit():
+    """A request's footprint is its cavity workspace plus allocation slack;
+    admission takes waiting requests in arrival order when they fit."""
+    import ctypes
+    import pyflow._hip as H
+    from pyflow.serving import CavityRequest, first_fit, footprint
+    L = H.lib()
+    for nx, ny in ((64, 64), (130, 97), (2048, 2048)):
+        req = CavityRequest(nx, ny, steps=10, Re=400.0)
+        p = req.params()
+        n = ctypes.c_size_t()
+        assert L.pf_batch_workspace_bytes(H.PREC_MIXED, 1, ctypes.byref(p), ctypes.byref(n)) == 0
+        assert footprint(req, "mixed") == n.value + 256
+        assert footprint(req, "fp64") > footprint(req, "mixed")
+    assert first_fit([5, 3, 4, 1], free_bytes=8, free_slots=4) == [0, 1]
+    assert first_fit([9, 3, 4, 1], free_bytes=8, free_slots=4) == [1, 2, 3]
+    assert first_fit([1, 1, 1], free_bytes=8, free_slots=2) == [0, 1]
+    assert first_fit([], free_bytes=8, free_slots=2) == []
+    assert first_fit([9], free_bytes=8, free_slots=2) == []
