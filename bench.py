@@ -39,7 +39,10 @@ def parse():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-accuracy", action="store_true", help="skip the float64 comparison run")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the float64 comparison runs")
+    ap.add_argument("--oracle-steps", type=int, default=0,
+                    help="steps from rest also compared with the FP64 NumPy oracle on the host (2 steps at "
+                         "1024x1024 take minutes of CPU time; default 0: skip)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 GPU form")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -113,6 +116,43 @@ def measured_traffic(n, precision, tile=None):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":", 1)[1])
     return None
+
+
+def oracle_accuracy(n, params, args):
+    """The first steps of the run (from rest) on the GPU in mixed precision
+    and with the FP64 NumPy oracle on the same inputs: field and continuity
+    differences. Step 0 from rest has a zero right-hand side; step 1 is the
+    first full projection."""
+    import numpy as np
+    from types import SimpleNamespace
+    from oracle import pyflow_oracle as O
+    from pyflow._device import DeviceBatch
+    k = args.oracle_steps
+    log(args, f"[bench] FP64 NumPy oracle, first {k} steps at {n}x{n}")
+    cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05,
+                          Re=1000.0, lid_velocity=1.0, lin_tol=1e-6, lin_maxiter=10000)
+    ref = O.allocate_state(n, n)
+    t0 = time.perf_counter()
+    for it in range(k):
+        res, _ = O.step(cfg, ref, it)
+    t_oracle = time.perf_counter() - t0
+    b = DeviceBatch([params], "mixed")
+    d = b.run(k)[0]
+    u = np.zeros((n + 2, n + 2)); v = np.zeros_like(u); p = np.zeros_like(u)
+    b.download(0, u, v, p)
+    b.close()
+    po = ref.fields["p"]
+    cont_o = float(res["continuity"])
+    return {
+        "steps": k,
+        "max_abs_du_over_U": float(np.max(np.abs(u - ref.fields["u"]))),
+        "max_abs_dv_over_U": float(np.max(np.abs(v - ref.fields["v"]))),
+        "rel_l2_dp": float(np.linalg.norm(p - po) / max(np.linalg.norm(po), 1e-300)),
+        "continuity": float(d.continuity),
+        "continuity_oracle": cont_o,
+        "continuity_diff": float(abs(d.continuity - cont_o)),
+        "oracle_seconds": round(t_oracle, 2),
+    }
 
 
 def cpu_baseline(n, iters_per_step, args):
@@ -366,9 +406,14 @@ def run_cavity(args):
     pcg_bytes = N * its * pcg_iter_bytes(s)
     avg_launch_s = (gpu_ms / 1e3) / max(rounds, 1)
     achieved = alg_bytes / max(rounds, 1) / avg_launch_s / 1e9
-    # PCG share of the GPU time: the mixed path runs one single-reduction round
-    # per PCG iteration, the float64 path two (SciPy's recurrence)
-    rounds_per_iter = 1.0 if args.precision == "mixed" else 2.0
+    # PCG share of the GPU time: the mixed path runs two PCG iterations per
+    # round (one with pcg_per_round = 1), the float64 path one iteration in two
+    # rounds (SciPy's recurrence)
+    if args.precision == "mixed":
+        two = (tile.pcg_per_round == 2) if tile is not None else os.environ.get("PYFLOW_CA", "1") != "0"
+        rounds_per_iter = 0.5 if two else 1.0
+    else:
+        rounds_per_iter = 2.0
     pcg_time_s = (gpu_ms / 1e3) * min(1.0, rounds_per_iter * its / max(rounds, 1))
     pcg_gbs = pcg_bytes / max(pcg_time_s, 1e-30) / 1e9
     value = N * args.steps / wall / 1e6
@@ -394,6 +439,9 @@ def run_cavity(args):
             "pcg_iterations_fp64": int(d64.rp_iterations_run),
         }
         log(args, f"[bench] accuracy {accuracy}")
+        if args.oracle_steps > 0:
+            accuracy["vs_oracle"] = oracle_accuracy(n, params, args)
+            log(args, f"[bench] accuracy vs the FP64 NumPy oracle {accuracy['vs_oracle']}")
 
     cpu = None
     if not args.no_cpu_baseline:
