@@ -85,6 +85,7 @@ struct pf_batch {
   bool any_xd = false;   // some cavity keeps its PCG iterate in FP64
   int xd_kinds = 0;      // rounds of those cavities: bit 0 one PCG iteration, bit 1 two
   bool multi = false;    // some cavity has several regions per workgroup
+  bool occ2 = false;     // every cavity fits the two-workgroups-per-CU kernel
   bool tables_dirty = true;
   int graph_len = 0;                      // rounds per captured graph (even; 0: direct launches)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by parity of the first round
@@ -248,6 +249,7 @@ static int sync_tables(pf_batch* b) {
   b->any_xd = false;
   b->xd_kinds = 0;
   b->multi = false;
+  b->occ2 = (b->prec == PF_PREC_MIXED) && env_switch("PYFLOW_OCC2", 1) != 0;
   for (int s = 0; s < b->max_cav; ++s)
     if (b->used[s]) {
       b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
@@ -255,6 +257,8 @@ static int sync_tables(pf_batch* b) {
       b->lds_floats = std::max(b->lds_floats, b->hcav[s].ca ? ca_lds_floats(tpb) : region_lds_floats(tpb));
       b->any_xd = b->any_xd || b->hcav[s].xd;
       if (b->hcav[s].xd) b->xd_kinds |= b->hcav[s].ca ? 2 : 1;
+      const PfCav& c = b->hcav[s];
+      b->occ2 = b->occ2 && c.ca && !c.xd && c.tpb <= 2 && c.rpw == 1;
       b->multi = b->multi || b->hcav[s].rpw > 1;
     }
   HIPCHK(hipMemcpyAsync(b->dcav, b->hcav.data(), sizeof(PfCav) * b->max_cav, hipMemcpyHostToDevice, b->stream));
@@ -328,7 +332,10 @@ static RoundLaunch round_launch(const pf_batch* b) {
   L.blk = dim3(TX, TY);
   L.grid = dim3(b->total_blocks);
   if (b->prec == PF_PREC_MIXED) {
-    L.fn = round_fn_u<float, 2>(b);  // U only shapes the FP64 P/Q loops
+    if (b->occ2)
+      L.fn = b->single_slot >= 0 ? (const void*)k_round_occ2<true> : (const void*)k_round_occ2<false>;
+    else
+      L.fn = round_fn_u<float, 2>(b);  // U only shapes the FP64 P/Q loops
     L.lds = (size_t)b->lds_floats * sizeof(float);
   } else {
     L.fn = b->unroll >= 4 ? round_fn_u<double, 4>(b) : b->unroll == 2 ? round_fn_u<double, 2>(b)
@@ -523,7 +530,8 @@ static int graph_len_of(int rounds) { return rounds <= 0 ? 0 : std::min(1024, st
 
 static int allow_region_lds() {
   const int bytes = std::max(region_lds_floats(8), ca_lds_floats(4)) * (int)sizeof(float);
-  const void* fns[10] = {
+  const void* fns[12] = {
+      (const void*)k_round_occ2<true>, (const void*)k_round_occ2<false>,
       (const void*)k_round<float, 2, true, 0, false>,  (const void*)k_round<float, 2, false, 0, false>,
       (const void*)k_round<float, 2, false, 1, false>, (const void*)k_round<float, 2, false, 2, false>,
       (const void*)k_round<float, 2, false, 3, false>, (const void*)k_round<float, 2, true, 0, true>,

@@ -358,7 +358,10 @@ __device__ __attribute__((always_inline)) inline void tail_ca(const PfCav& C, Pf
   S.phase = PH_CG;
 }
 
-__device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
+// V tells instances apart: each kernel instance that calls the cold functions
+// gets its own copies, compiled within that kernel's register budget.
+template <int V>
+__attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const double* tot) {
   const int ph = S.pending;
   S.pending = PH_IDLE;
   switch (ph) {
@@ -1488,7 +1491,7 @@ struct KArgs {
 // registers stay off the straight-line path of the PCG rounds.
 // Cg points at the cavity's descriptor in global memory (a kernel-argument
 // copy passed by reference would have to be spilled to scratch first).
-template <class T>
+template <class T, int V = 0>
 __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A, int b,
                                                      double (*sm)[NP], double* part_out) {
   const PfCav& C = *Cg;
@@ -1581,7 +1584,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
         else
           tail_cg(C, st, tot);
       } else {
-        tail(*Cg, st, tot);
+        tail<0>(*Cg, st, tot);
       }
     }
     __syncthreads();  // tail result visible
@@ -1657,7 +1660,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
 // partials are folded by one wave per slot: wave w sums slot w of every
 // workgroup (per lane: increasing workgroup index; then the DPP tree), so all
 // fourteen slots of the PCG round fold in parallel; thread 63 runs the tail.
-template <int M, bool MULTI, bool XD>
+template <int M, bool MULTI, bool XD, int V = 0>
 __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c,
                                                                     int b, PfScal& st, double (*sm)[NP], double* tot,
                                                                     float* lds) {
@@ -1703,7 +1706,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
       if (__builtin_expect(hot, 1))
         tail_ca(C, st, tot);
       else
-        tail(*Cg, st, tot);
+        tail<V>(*Cg, st, tot);
     }
     __syncthreads();  // tail result visible
   }
@@ -1752,7 +1755,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
     }
     block_partials_ca(acc, sm, part_out);
   } else {
-    round_cold<float>(phase, Cg, A, b, sm, part_out);
+    round_cold<float, V>(phase, Cg, A, b, sm, part_out);
   }
   if (b == 0) {
     __syncthreads();
@@ -1824,6 +1827,40 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   } else {
     round_body<T, U, 0, false, MULTI>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
   }
+}
+
+// Two workgroups per CU: batches whose cavities all run the two-iteration
+// FP32-iterate round on regions of at most two tiles, one region per
+// workgroup (LDS: 57.6 KB per workgroup). The register budget is 64 VGPRs
+// (8 waves per SIMD), so while one workgroup waits on a barrier or a load
+// the other computes. The cold phases are their own instances (V = 1),
+// compiled within the same budget.
+template <bool SINGLE>
+__global__ __attribute__((amdgpu_flat_work_group_size(NTH, NTH), amdgpu_waves_per_eu(8, 8))) void k_round_occ2(
+    const KArgs KA) {
+  __shared__ PfScal st;
+  __shared__ double sm[NWAVE][NP];
+  __shared__ double tot[NP];
+  extern __shared__ float lds_dyn[];  // ca_lds_floats(tpb) floats
+  const int g = blockIdx.x;
+  int c;
+  const PfCav* cp;
+  if constexpr (SINGLE) {
+    c = KA.slot0;
+    cp = &KA.cav0;
+  } else {
+    c = KA.blk_cav[g];
+    if (c < 0) return;
+    cp = KA.cavs + c;
+  }
+  const PfCav& C = *cp;
+  const PfCav* Cg = KA.cavs + c;
+  const int lg = g - C.blk0;
+  const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
+  if (C.tpb == 1)
+    round_body_ca<1, false, false, 1>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
+  else
+    round_body_ca<2, false, false, 1>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
 }
 
 // Standalone float64 operators (numerics API), one cell per thread.

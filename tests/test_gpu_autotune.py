@@ -109,3 +109,36 @@ def test_fp64_iterate_kinds_mixed_in_one_batch(monkeypatch):
                 O.step(cfg, ref, it)
             assert float(np.max(np.abs(um - ref.fields["u"]))) <= 1e-5
     mix.close()
+
+
+def test_two_workgroups_per_cu_kernel_equals_default(monkeypatch):
+    """A batch whose cavities all use regions of at most two tiles, one per
+    workgroup, runs the kernel instance with two workgroups per CU; it gives
+    the same bits as the one-workgroup-per-CU instance (PYFLOW_OCC2=0) and
+    follows the FP64 oracle."""
+    from pyflow import autotune as A
+    from pyflow._device import DeviceBatch
+    re, tol, maxiter = 400.0, 1e-10, 5000
+    specs = [(130, 97, A.Tile(2, 1024, 2)), (70, 40, A.Tile(1, 1024, 2)), (32, 32, A.Tile(0, 0, 0))]
+    ps = [A.apply(_cavity(x, y, re, tol, maxiter), t) for x, y, t in specs]
+    out = {}
+    for occ in ("1", "0"):
+        monkeypatch.setenv("PYFLOW_OCC2", occ)
+        b = DeviceBatch(ps, "mixed")
+        d = b.run(4)
+        fields = []
+        for k, (x, y, _) in enumerate(specs):
+            u = np.zeros((y + 2, x + 2)); v = np.zeros_like(u)
+            b.download(k, u, v, None)
+            fields.append((u, v, d[k].rp_iterations_run))
+        b.close()
+        out[occ] = fields
+    for (u1, v1, i1), (u0, v0, i0) in zip(out["1"], out["0"]):
+        assert np.array_equal(u1, u0) and np.array_equal(v1, v0) and i1 == i0
+    x, y, _ = specs[0]
+    cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05,
+                          Re=re, lid_velocity=1.0, lin_tol=tol, lin_maxiter=maxiter)
+    ref = O.allocate_state(x, y)
+    for it in range(4):
+        O.step(cfg, ref, it)
+    assert float(np.max(np.abs(out["1"][0][0] - ref.fields["u"]))) <= 1e-5
