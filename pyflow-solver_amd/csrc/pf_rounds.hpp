@@ -1383,8 +1383,8 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
 
 // Workgroup partials of the two-iteration round (KCA values per thread): a
 // transposed wave reduction. Each exchange pairs a lane with one whose index
-// differs in one more bit (lane distance 32 and 16 through the LDS crossbar,
-// then the DPP row mirror (index xor 15) and half-row mirror (xor 7)); the
+// differs in one more bit (lane distance 32 and 16 by the gfx950 permlane
+// swaps, then the DPP row mirror (index xor 15) and half-row mirror (xor 7)); the
 // lane keeps half of the values it carries and adds the partner's copy of
 // that half, so 16 values become 8, 4, 2, 1, and the quad sums (xor 2, xor 1)
 // finish. 17 FP64 additions per lane instead of 6 per value; lane 4q ends with
@@ -1396,6 +1396,18 @@ __device__ __attribute__((always_inline)) inline double dpp_full(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
+template <bool HALVES>
+__device__ __attribute__((always_inline)) inline double swap_sum(double x, double y) {
+  const unsigned long long xb = __double_as_longlong(x), yb = __double_as_longlong(y);
+  const unsigned xl = (unsigned)xb, xh = (unsigned)(xb >> 32), yl = (unsigned)yb, yh = (unsigned)(yb >> 32);
+  const auto lo = HALVES ? __builtin_amdgcn_permlane32_swap(xl, yl, false, false)
+                         : __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+  const auto hi = HALVES ? __builtin_amdgcn_permlane32_swap(xh, yh, false, false)
+                         : __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+  const double xs = __longlong_as_double((long long)((unsigned long long)lo[0] | ((unsigned long long)hi[0] << 32)));
+  const double ys = __longlong_as_double((long long)((unsigned long long)lo[1] | ((unsigned long long)hi[1] << 32)));
+  return xs + ys;
+}
 __device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], double* dst) {
   static_assert(KCA <= 16 && NP >= 16, "16 slots carried through the exchange");
   const int tid = tid_flat();
@@ -1403,24 +1415,14 @@ __device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], doub
 #pragma unroll
   for (int k = KCA; k < 16; ++k) v[k] = 0.0;
   double a[8], b[4], c[2];
-  {
-    const bool hi = lane & 32;
+  // Distances 32 and 16: v_permlane32_swap / v_permlane16_swap exchange the
+  // upper half (odd rows) of x with the lower half (even rows) of y, so each
+  // lane ends with its own kept value and its partner's copy of it, and the
+  // sum needs no select (FP64 addition is commutative).
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const double keep = hi ? v[8 + k] : v[k];
-      const double give = hi ? v[k] : v[8 + k];
-      a[k] = keep + __shfl_xor(give, 32);
-    }
-  }
-  {
-    const bool hi = lane & 16;
+  for (int k = 0; k < 8; ++k) a[k] = swap_sum<true>(v[k], v[8 + k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double keep = hi ? a[4 + k] : a[k];
-      const double give = hi ? a[k] : a[4 + k];
-      b[k] = keep + __shfl_xor(give, 16);
-    }
-  }
+  for (int k = 0; k < 4; ++k) b[k] = swap_sum<false>(a[k], a[4 + k]);
   {
     const bool hi = lane & 8;
 #pragma unroll
