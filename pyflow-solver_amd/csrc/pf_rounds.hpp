@@ -159,12 +159,16 @@ __device__ inline void wave_reduce_k(double (&v)[NP]) {
   for (int k = 0; k < K; ++k) v[k] = ((MM >> k) & 1u) ? wave_reduce<true>(v[k]) : wave_reduce<false>(v[k]);
 }
 
+// Partials are stored slot-major: slot k of workgroup b at part[k * nblk + b]
+// (per round parity), so a wave that folds one slot over all workgroups reads
+// consecutive doubles. A writer gets dst = &part[b] and stride = nblk.
+//
 // Workgroup partials of the first K of NP per-thread values: every wave
 // reduces on the DPP crossbar, lane 63 parks the wave total in sm[wave], and
 // after one barrier thread k < K folds the NWAVE totals in wave order and
-// stores slot k to dst[k].
+// stores slot k to dst[k * stride].
 template <int K, unsigned MM>
-__device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], double* dst) {
+__device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], double* dst, int stride) {
   const int tid = tid_flat();
   const int lane = tid & 63, wave = tid >> 6;
   wave_reduce_k<K, MM>(v);
@@ -178,7 +182,7 @@ __device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], doubl
     double t = sm[0][tid];
 #pragma unroll
     for (int w = 1; w < NWAVE; ++w) t = mx ? nanmax(t, sm[w][tid]) : t + sm[w][tid];
-    dst[tid] = t;
+    dst[(size_t)tid * stride] = t;
   }
 }
 
@@ -199,8 +203,8 @@ __device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], doubl
     default: break;                         \
   }
 
-__device__ inline void block_partials_phase(int ph, double (&v)[NP], double (*sm)[NP], double* dst) {
-#define PF_BP(K, MM) block_partials_k<K, MM>(v, sm, dst)
+__device__ inline void block_partials_phase(int ph, double (&v)[NP], double (*sm)[NP], double* dst, int stride) {
+#define PF_BP(K, MM) block_partials_k<K, MM>(v, sm, dst, stride)
   PF_PHASE_LAYOUT(PF_BP)
 #undef PF_BP
 }
@@ -1408,7 +1412,7 @@ __device__ __attribute__((always_inline)) inline double swap_sum(double x, doubl
   const double ys = __longlong_as_double((long long)((unsigned long long)lo[1] | ((unsigned long long)hi[1] << 32)));
   return xs + ys;
 }
-__device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], double* dst) {
+__device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], double* dst, int stride) {
   static_assert(KCA <= 16 && NP >= 16, "16 slots carried through the exchange");
   const int tid = tid_flat();
   const int lane = tid & 63, wave = tid >> 6;
@@ -1448,7 +1452,7 @@ __device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], doub
     double t = sm[0][tid];
 #pragma unroll
     for (int w = 1; w < NWAVE; ++w) t += sm[w][tid];
-    dst[tid] = t;
+    dst[(size_t)tid * stride] = t;
   }
 }
 
@@ -1495,7 +1499,7 @@ struct KArgs {
 // copy passed by reference would have to be spilled to scratch first).
 template <class T, int V = 0>
 __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A, int b,
-                                                     double (*sm)[NP], double* part_out) {
+                                                     double (*sm)[NP], double* part_out, int pstride) {
   const PfCav& C = *Cg;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < C.rpw; ++k) {
@@ -1509,7 +1513,7 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
       if (i < C.nx && j < C.ny) cell_phase<T>(phase, C, A, j, i, acc);
     }
   }
-  if (phase_nparts(phase) > 0) block_partials_phase(phase, acc, sm, part_out);
+  if (phase_nparts(phase) > 0) block_partials_phase(phase, acc, sm, part_out, pstride);
 }
 
 // Body of one round for workgroup b of cavity c. M > 0 (float32 only): the
@@ -1544,7 +1548,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
     const int q = lane + 64 * t;
     const size_t qq = (wf && q < nblk) ? (size_t)q : 0;
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[qq * NP + koff + k];
+    for (int k = 0; k < KMAX; ++k) pre[t][k] = prev[(size_t)(koff + k) * nblk + qq];
   }
   const uint32_t sw = src[tid < NW32 ? tid : 0];
   if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
@@ -1567,7 +1571,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
         }
       for (int q = lane + 64 * PRE; q < nblk; q += 64) {
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) pa[k] = redop(mm, k, pa[k], prev[(size_t)q * NP + koff + k]);
+        for (int k = 0; k < KMAX; ++k) pa[k] = redop(mm, k, pa[k], prev[(size_t)(koff + k) * nblk + q]);
       }
       if (__builtin_expect(hot, 1))
         wave_reduce_k<KMAX, 0u>(pa);
@@ -1619,7 +1623,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   A.alphaf = (float)A.alpha;
   A.minvf = (float)C.invd;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
+  double* part_out = C.part + (size_t)parity * nblk * NP + b;
   // Workgroup b owns regions b*rpw .. b*rpw+rpw-1 (fewer at the cavity's end);
   // without MULTI every cavity of the batch has rpw = 1.
   const int r0 = MULTI ? b * C.rpw : b;
@@ -1633,7 +1637,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
         region_cg_f32<M, XD>(C, A, r, acc, lds, L);
       }
     }
-    block_partials_k<(XD ? KXD : 3), 0u>(acc, sm, part_out);
+    block_partials_k<(XD ? KXD : 3), 0u>(acc, sm, part_out, nblk);
   } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
     for (int r = r0; r < r1; ++r) {
       const int ty0 = (r / C.tiles_x) * C.tpb;
@@ -1644,11 +1648,11 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
         for (int m = 0; m < C.tpb; m += U) chunk_q<T, U>(C, A, ti, acc);
     }
     if (phase == PH_P)
-      block_partials_k<1, 0u>(acc, sm, part_out);
+      block_partials_k<1, 0u>(acc, sm, part_out, nblk);
     else
-      block_partials_k<2, 0u>(acc, sm, part_out);
+      block_partials_k<2, 0u>(acc, sm, part_out, nblk);
   } else {
-    round_cold<T>(phase, Cg, A, b, sm, part_out);
+    round_cold<T>(phase, Cg, A, b, sm, part_out, nblk);
   }
   if (b == 0) {
     __syncthreads();
@@ -1681,7 +1685,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
 #pragma unroll
   for (int t = 0; t < PRE; ++t) {
     const int q = lane + 64 * t;
-    pre[t] = prev[(size_t)(q < nblk ? q : 0) * NP + wave];
+    pre[t] = prev[(size_t)wave * nblk + (q < nblk ? q : 0)];
   }
   const uint32_t sw = src[tid < NW32 ? tid : 0];
   if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
@@ -1697,7 +1701,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
       for (int t = 0; t < PRE; ++t)
         if (lane + 64 * t < nblk) v = mx ? nanmax(v, pre[t]) : v + pre[t];
       for (int q = lane + 64 * PRE; q < nblk; q += 64) {
-        const double w = prev[(size_t)q * NP + wave];
+        const double w = prev[(size_t)wave * nblk + q];
         v = mx ? nanmax(v, w) : v + w;
       }
       v = mx ? wave_reduce<true>(v) : wave_reduce<false>(v);
@@ -1747,7 +1751,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   A.ca0 = st.ca0;
   A.ca1 = st.ca1;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  double* part_out = C.part + ((size_t)parity * nblk + b) * NP;
+  double* part_out = C.part + (size_t)parity * nblk * NP + b;
   const int r0 = MULTI ? b * C.rpw : b;
   const int r1 = MULTI ? min(r0 + C.rpw, C.nreg) : b + 1;
   if (__builtin_expect(phase == PH_CG, 1)) {
@@ -1755,9 +1759,9 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
       if (r > r0) __syncthreads();  // the previous region's LDS reads are done
       region_ca<M, XD, MULTI>(C, A, r, acc, lds);
     }
-    block_partials_ca(acc, sm, part_out);
+    block_partials_ca(acc, sm, part_out, nblk);
   } else {
-    round_cold<float, V>(phase, Cg, A, b, sm, part_out);
+    round_cold<float, V>(phase, Cg, A, b, sm, part_out, nblk);
   }
   if (b == 0) {
     __syncthreads();
