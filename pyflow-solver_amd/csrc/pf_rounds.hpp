@@ -1751,17 +1751,18 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   A.ca0 = st.ca0;
   A.ca1 = st.ca1;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  double* part_out = C.part + (size_t)parity * nblk * NP + b;
   const int r0 = MULTI ? b * C.rpw : b;
   const int r1 = MULTI ? min(r0 + C.rpw, C.nreg) : b + 1;
+  // The partials' address is formed after the region loop (from the
+  // descriptor again), which keeps it out of the loop's registers.
   if (__builtin_expect(phase == PH_CG, 1)) {
     for (int r = r0; r < r1; ++r) {
       if (r > r0) __syncthreads();  // the previous region's LDS reads are done
       region_ca<M, XD, MULTI>(C, A, r, acc, lds);
     }
-    block_partials_ca(acc, sm, part_out, nblk);
+    block_partials_ca(acc, sm, C.part + (size_t)parity * C.nblk * NP + b, C.nblk);
   } else {
-    round_cold<float, V>(phase, Cg, A, b, sm, part_out, nblk);
+    round_cold<float, V>(phase, Cg, A, b, sm, C.part + (size_t)parity * C.nblk * NP + b, C.nblk);
   }
   if (b == 0) {
     __syncthreads();
