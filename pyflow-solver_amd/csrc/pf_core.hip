@@ -210,6 +210,9 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   c.growth = p->cfl_growth;
   c.tol = p->lin_tol;
   c.invd = p->inv_diag;
+  c.cxf = (float)(1.0 / (p->dx * p->dx));
+  c.cyf = (float)(1.0 / (p->dy * p->dy));
+  c.invdf = (float)p->inv_diag;
   c.inner_rtol = (p->inner_rtol > 0.0) ? p->inner_rtol : 1e-4;
   // MIXED: accumulate the PCG iterate in FP64 on large grids. Rounding the
   // iterate to FP32 perturbs the true residual by about eps32 * cond(A) of the

@@ -83,11 +83,12 @@ struct PfCav {
   int scheme, lid_on, jacobi, maxiter, op, mode, prec;
   int xd;  // MIXED: the PCG iterate is accumulated in FP64 (x64) instead of FP32
   int ca;  // MIXED: two PCG iterations per round (s-step basis, 4-cell halo)
+  float cxf, cyf, invdf;  // (float)(1/dx^2), (float)(1/dy^2), (float)invd, formed on the host
   double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
   void *u, *v, *p, *us, *vs, *r, *d0, *d1, *x;  // element type T
   void* r1;                                     // MIXED: r double-buffered (r, r1); p in d0/d1
   double *b, *x64;                              // x64 aliases x in FP64 mode
-  double* part;                                 // [2][nblk][NP]
+  double* part;                                 // [2][NP][nblk]: slot k of workgroup b at k*nblk + b
 };
 
 // Scalar state of one cavity; two copies (round parity) per cavity.
@@ -1498,9 +1499,13 @@ struct KArgs {
 // Cg points at the cavity's descriptor in global memory (a kernel-argument
 // copy passed by reference would have to be spilled to scratch first).
 template <class T, int V = 0>
-__attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A, int b,
+__attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A0, int b,
                                                      double (*sm)[NP], double* part_out, int pstride) {
   const PfCav& C = *Cg;
+  PhaseArgs A = A0;
+  // FIN: p = x - x[0] (pressure_solver.py:147-149); the solution is b itself
+  // when the right-hand side was zero.
+  if (phase == PH_FIN) A.x0 = A.x_is_b ? C.b[0] : C.x64[0];
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < C.rpw; ++k) {
     const int r = b * C.rpw + k;
@@ -1615,13 +1620,12 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   A.it = st.it;
   A.x_is_b = st.x_is_b;
   A.par = parity;
-  A.x0 = 0.0;
-  if (phase == PH_FIN) A.x0 = st.x_is_b ? C.b[0] : C.x64[0];
-  A.cx = (float)(1.0 / (C.dx * C.dx));
-  A.cy = (float)(1.0 / (C.dy * C.dy));
+  A.x0 = 0.0;  // FIN reads x[0] in round_cold
+  A.cx = C.cxf;
+  A.cy = C.cyf;
   A.betaf = (float)A.beta;
   A.alphaf = (float)A.alpha;
-  A.minvf = (float)C.invd;
+  A.minvf = C.invdf;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
   double* part_out = C.part + (size_t)parity * nblk * NP + b;
   // Workgroup b owns regions b*rpw .. b*rpw+rpw-1 (fewer at the cavity's end);
@@ -1736,13 +1740,12 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   A.it = st.it;
   A.x_is_b = st.x_is_b;
   A.par = parity;
-  A.x0 = 0.0;
-  if (phase == PH_FIN) A.x0 = st.x_is_b ? C.b[0] : C.x64[0];
-  A.cx = (float)(1.0 / (C.dx * C.dx));
-  A.cy = (float)(1.0 / (C.dy * C.dy));
+  A.x0 = 0.0;  // FIN reads x[0] in round_cold
+  A.cx = C.cxf;
+  A.cy = C.cyf;
   A.betaf = (float)A.beta;
   A.alphaf = (float)A.alpha;
-  A.minvf = (float)C.invd;
+  A.minvf = C.invdf;
   A.ca0f = (float)st.ca0;
   A.cb0f = (float)st.cb0;
   A.ca1f = (float)st.ca1;
