@@ -1160,12 +1160,7 @@ __device__ __attribute__((always_inline)) inline CaCell ca_steps(const PhaseArgs
 
 // XD: the iterate is the FP64 x64 (x64 += alpha_k p_k + alpha_k+1 p_k+1 in
 // FP64), as in the one-iteration FP64-iterate round.
-// SPARSE: ghost copies are written only by the cells that have ghosts. A
-// workgroup that walks several regions is bound by LDS bandwidth and gains
-// (4096^2: 207 instead of 217 us per round); a single region per workgroup is
-// bound by its latency chain, where the branch costs more than the stores
-// (1024^2: 17.0 instead of 16.8 us).
-template <int M, bool XD, bool SPARSE>
+template <int M, bool XD>
 __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, const PhaseArgs& A, int r, double* acc,
                                                                 float* __restrict__ lds) {
   const int nx = C.nx, ny = C.ny;
@@ -1206,9 +1201,9 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
   const bool pinned = (C.op == 0);
   const float scx = pinned ? -A.cx : A.cx, scy = pinned ? -A.cy : A.cy;
   // Cell code: LDS offsets of the (at most two) ghosts the cell writes, the
-  // cell's own offset standing in for a missing one (rewriting the same value
-  // there is harmless), so every write can be three plain stores; bit 30
-  // marks a cell with a ghost, bit 31 an identity row.
+  // cell's own offset standing in for a missing one; bit 30 marks a cell with
+  // a ghost (only those write the copies: 4096^2 207 instead of 217 us per
+  // round, 1024^2 13.7 instead of 14.2 us), bit 31 an identity row.
   auto code_of = [&](int gj, int gi, int row, int col) -> unsigned {
     const int o = row * HQ + col;
     int g1 = o, g2 = o;
@@ -1217,13 +1212,12 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
     if (gj == ny - 1 && row + 1 < NR) g2 = o + HQ;
     if (gj == 0 && row >= 1) g2 = o - HQ;
     const bool id = pinned ? (gj == 0 && gi == 0) : (gj == 0 || gj == ny - 1 || gi == 0 || gi == nx - 1);
-    return (unsigned)g1 | ((unsigned)g2 << 15) | (id ? 0x80000000u : 0u) |
-           ((SPARSE && (g1 != o || g2 != o)) ? 0x40000000u : 0u);
+    return (unsigned)g1 | ((unsigned)g2 << 15) | (id ? 0x80000000u : 0u) | ((g1 != o || g2 != o) ? 0x40000000u : 0u);
   };
   static_assert(NR * HQ < (1 << 15), "LDS offsets fit 15 bits");
   auto put = [&](float* L, int o, unsigned code, float v) {
     L[o] = v;
-    if (!SPARSE || (code & 0x40000000u)) {
+    if (code & 0x40000000u) {
       L[code & 0x7fffu] = v;
       L[(code >> 15) & 0x7fffu] = v;
     }
@@ -1761,7 +1755,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   if (__builtin_expect(phase == PH_CG, 1)) {
     for (int r = r0; r < r1; ++r) {
       if (r > r0) __syncthreads();  // the previous region's LDS reads are done
-      region_ca<M, XD, MULTI>(C, A, r, acc, lds);
+      region_ca<M, XD>(C, A, r, acc, lds);
     }
     block_partials_ca(acc, sm, C.part + (size_t)parity * C.nblk * NP + b, C.nblk);
   } else {
