@@ -210,7 +210,7 @@ int pf_batch_timing(pf_batch* b, double* gpu_ms, int64_t* rounds);
  * queues every round with its own launch). Replays and direct launches run
  * the same kernels with the same arguments, so results are bitwise equal.
  * The graphs are rebuilt after any change of the batch's cavities. Default:
- * PYFLOW_GRAPH if set, else 16. The reference has no counterpart (its step
+ * PYFLOW_GRAPH if set, else 64. The reference has no counterpart (its step
  * runs on the host: solvers/solver.py:15); this serves the batched and
  * served paths (drivers/simulation_driver.py:58-110 per cavity). */
 int pf_batch_set_graph(pf_batch* b, int rounds);

@@ -558,7 +558,7 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->stream = (hipStream_t)stream;
   b->hcav.assign(max_cav, PfCav{});
   b->unroll = env_int("PYFLOW_UNROLL", 2);
-  b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 16));
+  b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 64));
   b->used.assign(max_cav, 0);
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dcav, sizeof(PfCav) * max_cav);
