@@ -196,8 +196,9 @@ def cpu_baseline(n, iters_per_step, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); this build has no multi-GPU path. "
-              "Independent cavities shard by request across GPUs; see DESIGN.md.", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); there is no multi-GPU path. "
+              "Independent cavities would shard by request, one process per GPU; a single large cavity by "
+              "row slabs with a two-row halo exchange per PCG round.", file=sys.stderr)
         return 2
     import torch
     torch.cuda.set_device(0)
