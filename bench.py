@@ -619,13 +619,25 @@ def batch_params(count, seed, inner_rtol):
     return params, ns
 
 
-# The 256-cavity batch: the autotuner picks the per-cavity workgroup cap for
-# the whole batch (several regions per workgroup on the larger cavities, so a
-# round holds fewer workgroups, each paying its state round trip and tail once).
-# Without it: at most 16 workgroups per cavity (+10% over 256, measured).
+# The 256-cavity batch: the autotuner picks the region height and the
+# per-cavity workgroup cap for the whole batch (two-tile regions, one per
+# workgroup, run two workgroups per CU; several regions per workgroup make a
+# round hold fewer workgroups, each paying its state round trip and tail once).
+# Without it: the default regions and at most 16 workgroups per cavity.
 # Served requests keep the default (256): few cavities run at a time there, and
 # a large one capped at 16 workgroups would leave most CUs idle.
 MANY_CAVITY_MAXWG = 16
+
+
+def batch_round_kernel(ns, rows, cap, precision):
+    """Round kernel the library launches for a mixed-precision batch of n x n
+    cavities below 1536 cells a side: the two-workgroups-per-CU instance when
+    every cavity's regions hold at most two tiles, one region per workgroup."""
+    from pyflow import autotune
+    blk = [autotune.blocking(int(n), int(n), rows or 4, cap) for n in ns]
+    if precision == "mixed" and all(tpb <= 2 and rpw == 1 for tpb, _, rpw, _ in blk):
+        return "pf::k_round_occ2<false>"
+    return "pf::k_round<float, U, false>"
 
 
 def run_batch(args):
@@ -636,12 +648,14 @@ def run_batch(args):
     steps = args.steps
     params, ns = batch_params(256, 0, args.inner_rtol)
     cells = int(np.sum(ns.astype(np.int64) ** 2))
-    cap, tuned = MANY_CAVITY_MAXWG, None
+    rows, cap, tuned = 0, MANY_CAVITY_MAXWG, None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        cap, tuned = autotune.tune_batch(params, args.precision, log=lambda m: log(args, m))
-        log(args, f"[bench] autotune batch: max_workgroups={cap} ({time.perf_counter() - t0:.1f} s)")
+        (rows, cap), tuned = autotune.tune_batch(params, args.precision, log=lambda m: log(args, m))
+        log(args, f"[bench] autotune batch: tile_rows={rows} max_workgroups={cap} "
+                  f"({time.perf_counter() - t0:.1f} s)")
     for p in params:
+        p.tile_rows = rows
         p.max_workgroups = cap
     batch = DeviceBatch(params, args.precision)
     if args.warmup > 0:
@@ -679,11 +693,11 @@ def run_batch(args):
                 "seed 0",
         "config": {"workload": f"256 cavities, QUICK, tol 1e-6, lin_maxiter 10000, {steps} steps", "cells": cells,
                    "precision": args.precision, "inner_rtol": args.inner_rtol,
-                   "max_workgroups_per_cavity": cap,
+                   "tile_rows": rows, "max_workgroups_per_cavity": cap,
                    "autotune_ps_per_cell_iteration": None if tuned is None else
-                   {str(k): round(v * 1e6, 4) for k, v in tuned.items()}},
-        "roofline": gpu_roofline("pf::k_round<float, U, false> (all cavities' regions in one launch)", alg_bytes,
-                                 gpu_ms, rounds),
+                   {f"rows{k[0]}_cap{k[1]}": round(v, 4) for k, v in tuned.items()}},
+        "roofline": gpu_roofline(f"{batch_round_kernel(ns, rows, cap, args.precision)} "
+                                 "(all cavities' regions in one launch)", alg_bytes, gpu_ms, rounds),
         "pcg": {"iterations_total": int(its.sum()), "mean_per_cavity_step": round(float(its.mean()) / steps, 1)},
         "nan_cavities": nan,
         "cpu_baseline": None,

@@ -221,18 +221,21 @@ def tune(nx: int, ny: int, precision: str | int = "mixed", *, warmup: int = 16, 
 
 
 def tune_batch(params: Sequence[H.CavityParams], precision: str | int = "mixed", *,
-               caps: Iterable[int] = (4, 8, 16, 32, 64, 256), warmup: int = 8, rounds: int = 24,
-               log=None) -> tuple[int, dict[int, float]]:
-    """Per-cavity workgroup cap that makes a whole batch's PCG rounds fastest.
+               candidates: Iterable[tuple[int, int]] = ((4, 16), (4, 64), (4, 256), (2, 64), (2, 1024)),
+               warmup: int = 8, rounds: int = 24, log=None) -> tuple[tuple[int, int], dict[tuple[int, int], float]]:
+    """Region height and per-cavity workgroup cap that make a whole batch's
+    PCG rounds fastest.
 
     Every cavity of ``params`` (their sizes; other fields are ignored) runs
-    the pinned Poisson solve together in one batch per candidate cap. Returns
-    the best cap and the GPU microseconds per cell-iteration of each cap."""
+    the pinned Poisson solve together in one batch per candidate
+    ``(tile_rows, max_workgroups)``. Regions of at most two tiles with one
+    region per workgroup let the library run two workgroups per CU. Returns
+    the best candidate and the GPU picoseconds per cell-iteration of each."""
     prec = precision_code(precision)
     sizes = [(int(p.nx), int(p.ny)) for p in params]
-    results: dict[int, float] = {}
-    for cap in caps:
-        ps = [_solve_params(nx, ny, prec, Tile(0, cap, 0), 1 << 30) for nx, ny in sizes]
+    results: dict[tuple[int, int], float] = {}
+    for rows, cap in candidates:
+        ps = [_solve_params(nx, ny, prec, Tile(rows, cap, 0), 1 << 30) for nx, ny in sizes]
         b = DeviceBatch(ps, prec)
         try:
             for s, (nx, ny) in enumerate(sizes):
@@ -242,8 +245,9 @@ def tune_batch(params: Sequence[H.CavityParams], precision: str | int = "mixed",
             b.close()
         # cells advanced per iteration differ by cavity: weight by the mean cavity size
         cells = sum(nx * ny for nx, ny in sizes) / len(sizes)
-        results[cap] = 1e3 * ms / (its * cells) if its > 0 else float("inf")
+        results[(rows, cap)] = 1e9 * ms / (its * cells) if its > 0 else float("inf")
         if log:
-            log(f"[autotune] batch of {len(sizes)} cap={cap}: {results[cap] * 1e6:.3f} ps/cell-iteration")
+            log(f"[autotune] batch of {len(sizes)} tile_rows={rows} cap={cap}: "
+                f"{results[(rows, cap)]:.3f} ps/cell-iteration")
     best = min(results, key=results.get)
     return best, results

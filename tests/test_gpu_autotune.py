@@ -75,9 +75,9 @@ def test_autotune_returns_a_measured_candidate(tmp_path, monkeypatch):
     assert A.cached(nx, ny, "mixed") == t
     A._cache.clear()
     assert A.cached(nx, ny, "mixed") == t  # from the JSON file
-    cap, res = A.tune_batch([_cavity(96, 96, 100.0, 1e-6, 100), _cavity(40, 72, 100.0, 1e-6, 100)], "mixed",
-                            caps=(2, 16), warmup=4, rounds=8)
-    assert cap in (2, 16) and all(np.isfinite(v) and v > 0 for v in res.values())
+    best, res = A.tune_batch([_cavity(96, 96, 100.0, 1e-6, 100), _cavity(40, 72, 100.0, 1e-6, 100)], "mixed",
+                             candidates=((4, 2), (4, 16), (2, 1024)), warmup=4, rounds=8)
+    assert best in ((4, 2), (4, 16), (2, 1024)) and all(np.isfinite(v) and v > 0 for v in res.values())
 
 
 def test_fp64_iterate_kinds_mixed_in_one_batch(monkeypatch):
