@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiles instead of timing the candidates first")
+    ap.add_argument("--tile", default=None, metavar="ROWS,CAP",
+                    help="batch: region height (tiles) and per-cavity workgroup cap for every cavity, instead "
+                         "of the autotuner's pick")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a plain 256 MiB device copy (10x) to calibrate PMC byte counters")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
@@ -101,6 +104,27 @@ TRAFFIC_PROFILES = {
     4096: os.path.join(ROOT, "profiles", "k_round_poisson_4096_pmc_traffic.md"),
 }
 PROFILED_TILE = (4, 256, 2)
+# The 256-cavity batch's summaries, taken with 2-tile regions and a cap of
+# 1024 workgroups per cavity (one region per workgroup).
+BATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_round_occ2_batch256_pmc_traffic.md")
+PROFILED_BATCH_TILE = (2, 1024)
+
+
+def traffic_line(path):
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
+def batch_traffic(precision, rows, cap):
+    """Median HBM bytes per round launch of the 256-cavity batch, measured
+    with the profiled tiling; None for other tilings or precisions."""
+    if precision != "mixed" or (rows, cap) != PROFILED_BATCH_TILE:
+        return None
+    return traffic_line(BATCH_TRAFFIC_PROFILE)
 
 
 def measured_traffic(n, precision, tile=None):
@@ -112,10 +136,7 @@ def measured_traffic(n, precision, tile=None):
         return None
     if tile is not None and (tile.tile_rows, tile.max_workgroups, tile.pcg_per_round) != PROFILED_TILE:
         return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":", 1)[1])
-    return None
+    return traffic_line(path)
 
 
 def oracle_accuracy(n, params, args):
@@ -649,7 +670,9 @@ def run_batch(args):
     params, ns = batch_params(256, 0, args.inner_rtol)
     cells = int(np.sum(ns.astype(np.int64) ** 2))
     rows, cap, tuned = 0, MANY_CAVITY_MAXWG, None
-    if not args.no_autotune:
+    if args.tile:
+        rows, cap = (int(v) for v in args.tile.split(","))
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         (rows, cap), tuned = autotune.tune_batch(params, args.precision, log=lambda m: log(args, m))
         log(args, f"[bench] autotune batch: tile_rows={rows} max_workgroups={cap} "
@@ -697,7 +720,8 @@ def run_batch(args):
                    "autotune_ps_per_cell_iteration": None if tuned is None else
                    {f"rows{k[0]}_cap{k[1]}": round(v, 4) for k, v in tuned.items()}},
         "roofline": gpu_roofline(f"{batch_round_kernel(ns, rows, cap, args.precision)} "
-                                 "(all cavities' regions in one launch)", alg_bytes, gpu_ms, rounds),
+                                 "(all cavities' regions in one launch)", alg_bytes, gpu_ms, rounds,
+                                 batch_traffic(args.precision, rows, cap)),
         "pcg": {"iterations_total": int(its.sum()), "mean_per_cavity_step": round(float(its.mean()) / steps, 1)},
         "nan_cavities": nan,
         "cpu_baseline": None,
