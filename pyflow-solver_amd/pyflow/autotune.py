@@ -17,7 +17,8 @@ candidate on the device: a pinned-Neumann Poisson solve (the operator of
 right-hand side runs a few warm-up rounds, then a timed span of rounds, and the
 candidate's cost is GPU time per PCG iteration executed in that span. The
 tolerance is set so the solve cannot finish inside the span. ``tune_batch``
-does the same for the per-cavity workgroup cap of a whole batch.
+does the same for the region height and per-cavity workgroup cap of a whole
+batch.
 
 Results are cached per (nx, ny, precision, device) in-process and, when
 ``PYFLOW_TUNE_CACHE`` names a file, in that JSON file. ``pyflow.autotune`` has
@@ -221,20 +222,21 @@ def tune(nx: int, ny: int, precision: str | int = "mixed", *, warmup: int = 16, 
 
 
 def tune_batch(params: Sequence[H.CavityParams], precision: str | int = "mixed", *,
-               candidates: Iterable[tuple[int, int]] = ((4, 16), (4, 64), (4, 256), (2, 64), (2, 1024)),
+               tilings: Iterable[tuple[int, int]] = ((4, 16), (4, 64), (4, 256), (2, 64), (2, 1024)),
                warmup: int = 8, rounds: int = 24, log=None) -> tuple[tuple[int, int], dict[tuple[int, int], float]]:
     """Region height and per-cavity workgroup cap that make a whole batch's
     PCG rounds fastest.
 
     Every cavity of ``params`` (their sizes; other fields are ignored) runs
-    the pinned Poisson solve together in one batch per candidate
-    ``(tile_rows, max_workgroups)``. Regions of at most two tiles with one
-    region per workgroup let the library run two workgroups per CU. Returns
-    the best candidate and the GPU picoseconds per cell-iteration of each."""
+    the pinned Poisson solve together in one batch per tiling
+    ``(tile_rows, max_workgroups)`` of ``tilings``. Regions of at most two
+    tiles with one region per workgroup let the library run two workgroups
+    per CU. Returns the best tiling and the GPU picoseconds per
+    cell-iteration of each."""
     prec = precision_code(precision)
     sizes = [(int(p.nx), int(p.ny)) for p in params]
     results: dict[tuple[int, int], float] = {}
-    for rows, cap in candidates:
+    for rows, cap in tilings:
         ps = [_solve_params(nx, ny, prec, Tile(rows, cap, 0), 1 << 30) for nx, ny in sizes]
         b = DeviceBatch(ps, prec)
         try:

@@ -76,7 +76,7 @@ def test_autotune_returns_a_measured_candidate(tmp_path, monkeypatch):
     A._cache.clear()
     assert A.cached(nx, ny, "mixed") == t  # from the JSON file
     best, res = A.tune_batch([_cavity(96, 96, 100.0, 1e-6, 100), _cavity(40, 72, 100.0, 1e-6, 100)], "mixed",
-                             candidates=((4, 2), (4, 16), (2, 1024)), warmup=4, rounds=8)
+                             tilings=((4, 2), (4, 16), (2, 1024)), warmup=4, rounds=8)
     assert best in ((4, 2), (4, 16), (2, 1024)) and all(np.isfinite(v) and v > 0 for v in res.values())
 
 
