@@ -1,10 +1,30 @@
-"""PyFlow CLI (reference: cli.py:35-194): ``pyflow run | validate | show-config``.
+"""Command line front end: ``pyflow run | validate | show-config``.
 
-Same flags, defaults, exit codes (2 invalid config, 3 restart hash mismatch,
-130 interrupt) and ``--json-stream`` line schema:
-``{"type":"step","iteration","dt","CFL","continuity","wall_time",
-"residuals":{Ru,Rv,Rp,continuity},"diagnostics":{...},"run_meta":{seed}?}``.
-Added: ``--precision {fp64,mixed}`` selects the GPU arithmetic (not hashed).
+Interface kept from the reference's ``cli.py:35-194``, which the dashboard and
+the AI controller drive through a subprocess:
+
+* Flags and defaults (all three commands): ``--nx 64 --ny 64 --re 100
+  --lid-velocity 1 --scheme {quick,upwind}=quick --disable-advection --cfl 0.5
+  --cfl-growth 1.05 --lin-tol 1e-10 --lin-maxiter 300 --checkpoint-interval 0
+  (0 = none) --log-jsonl PATH --seed N --diagnostics``; ``run`` adds ``--steps
+  100`` (an absolute bound on the iteration index), ``--checkpoint``,
+  ``--restart``, ``--allow-hash-mismatch``, ``--json-stream``,
+  ``--continuity-threshold``, ``--progress``, ``--assert-invariants``.
+* Exit codes: 0 done, 2 invalid configuration, 3 restart refused because the
+  checkpoint's config hash differs, 130 interrupted (after a best-effort final
+  checkpoint).
+* ``--json-stream`` line, one per step, compact separators::
+
+      {"type":"step","iteration":..,"dt":..,"CFL":..,"continuity":..,
+       "wall_time":..,"residuals":{"Ru","Rv","Rp","continuity"},
+       "diagnostics":{every diagnostics key except iteration, dt, CFL,
+       wall_time},"run_meta":{"seed":..}}   (run_meta only with --seed)
+
+* A restart resumes at checkpoint iteration + 1; an unreadable checkpoint
+  prints a message and starts from rest.
+
+Added here: ``--precision {fp64,mixed}`` picks the GPU arithmetic of the step
+(not part of the config hash).
 """
 from __future__ import annotations
 
@@ -12,153 +32,198 @@ import argparse
 import json
 import os
 import sys
+from typing import Any
 
 from pydantic import ValidationError
 
 from .config.model import ConfigError, SimulationConfig
-from .core.ghost_fields import allocate_state
+from .core.ghost_fields import State, allocate_state
 from .drivers.simulation_driver import SimulationDriver
 from .residuals.manager import ResidualManager
 
-__all__ = ["build_parser", "main"]
+__all__ = ["build_parser", "main", "stream_record"]
+
+EXIT_OK, EXIT_BAD_CONFIG, EXIT_HASH_MISMATCH, EXIT_INTERRUPTED = 0, 2, 3, 130
+
+# (flag, argparse keyword arguments): options every command accepts.
+_MODEL_OPTIONS: tuple[tuple[str, dict[str, Any]], ...] = (
+    ("--nx", {"type": int, "default": 64}),
+    ("--ny", {"type": int, "default": 64}),
+    ("--re", {"type": float, "default": 100.0}),
+    ("--lid-velocity", {"type": float, "default": 1.0}),
+    ("--scheme", {"choices": ["quick", "upwind"], "default": "quick"}),
+    ("--disable-advection", {"action": "store_true"}),
+    ("--cfl", {"type": float, "default": 0.5}),
+    ("--cfl-growth", {"type": float, "default": 1.05}),
+    ("--lin-tol", {"type": float, "default": 1e-10}),
+    ("--lin-maxiter", {"type": int, "default": 300}),
+    ("--checkpoint-interval", {"type": int, "default": 0}),
+    ("--log-jsonl", {"type": str, "default": None}),
+    ("--seed", {"type": int, "default": None}),
+    ("--diagnostics", {"action": "store_true"}),
+    ("--precision", {"choices": ["fp64", "mixed"], "default": None}),
+)
+
+# Options of ``run`` only.
+_RUN_OPTIONS: tuple[tuple[str, dict[str, Any]], ...] = (
+    ("--steps", {"type": int, "default": 100}),
+    ("--checkpoint", {"type": str, "default": None}),
+    ("--restart", {"type": str, "default": None}),
+    ("--allow-hash-mismatch", {"action": "store_true"}),
+    ("--json-stream", {"action": "store_true"}),
+    ("--continuity-threshold", {"type": float, "default": None}),
+    ("--progress", {"action": "store_true"}),
+    ("--assert-invariants", {"action": "store_true"}),
+)
+
+_COMMANDS = (
+    ("run", "Run a simulation", True),
+    ("validate", "Validate and display normalized config", False),
+    ("show-config", "Show normalized config (hash only)", False),
+)
+
+# Diagnostics keys that the stream line carries at its top level instead.
+_TOP_LEVEL_KEYS = frozenset({"iteration", "dt", "CFL", "wall_time"})
 
 
 def build_parser() -> argparse.ArgumentParser:
-    p = argparse.ArgumentParser(description="PyFlow CLI")
-    sub = p.add_subparsers(dest="command", required=True)
+    parser = argparse.ArgumentParser(description="PyFlow CLI")
+    commands = parser.add_subparsers(dest="command", required=True)
+    for name, help_text, is_run in _COMMANDS:
+        sp = commands.add_parser(name, help=help_text)
+        for flag, spec in _MODEL_OPTIONS + (_RUN_OPTIONS if is_run else ()):
+            sp.add_argument(flag, **spec)
+    return parser
 
-    def add_common(sp: argparse.ArgumentParser):
-        sp.add_argument("--nx", type=int, default=64)
-        sp.add_argument("--ny", type=int, default=64)
-        sp.add_argument("--re", type=float, default=100.0)
-        sp.add_argument("--lid-velocity", type=float, default=1.0)
-        sp.add_argument("--scheme", choices=["quick", "upwind"], default="quick")
-        sp.add_argument("--disable-advection", action="store_true")
-        sp.add_argument("--cfl", type=float, default=0.5)
-        sp.add_argument("--cfl-growth", type=float, default=1.05)
-        sp.add_argument("--lin-tol", type=float, default=1e-10)
-        sp.add_argument("--lin-maxiter", type=int, default=300)
-        sp.add_argument("--checkpoint-interval", type=int, default=0)
-        sp.add_argument("--log-jsonl", type=str, default=None)
-        sp.add_argument("--seed", type=int, default=None)
-        sp.add_argument("--diagnostics", action="store_true")
-        sp.add_argument("--precision", choices=["fp64", "mixed"], default=None)
 
-    run_p = sub.add_parser("run", help="Run a simulation")
-    add_common(run_p)
-    run_p.add_argument("--steps", type=int, default=100)
-    run_p.add_argument("--checkpoint", type=str, default=None)
-    run_p.add_argument("--restart", type=str, default=None)
-    run_p.add_argument("--allow-hash-mismatch", action="store_true")
-    run_p.add_argument("--json-stream", action="store_true")
-    run_p.add_argument("--continuity-threshold", type=float, default=None)
-    run_p.add_argument("--progress", action="store_true")
-    run_p.add_argument("--assert-invariants", action="store_true")
+def _interval(ns: argparse.Namespace) -> int | None:
+    """--checkpoint-interval with 0 meaning "no periodic checkpoints"."""
+    return ns.checkpoint_interval or None
 
-    val_p = sub.add_parser("validate", help="Validate and display normalized config")
-    add_common(val_p)
-    show_p = sub.add_parser("show-config", help="Show normalized config (hash only)")
-    add_common(show_p)
-    return p
+
+def _config(ns: argparse.Namespace) -> SimulationConfig:
+    return SimulationConfig(
+        nx=ns.nx, ny=ns.ny, Re=ns.re, lid_velocity=ns.lid_velocity,
+        advection_scheme=ns.scheme, disable_advection=ns.disable_advection,
+        cfl_target=ns.cfl, cfl_growth=ns.cfl_growth,
+        lin_tol=ns.lin_tol, lin_maxiter=ns.lin_maxiter,
+        checkpoint_interval=_interval(ns), log_path=ns.log_jsonl,
+        diagnostics=ns.diagnostics, seed=ns.seed, turbulence_model=None,
+    )
+
+
+def stream_record(residuals: dict[str, Any], diag: dict[str, Any], seed: int | None = None) -> dict[str, Any]:
+    """The ``--json-stream`` object of one step."""
+    rec: dict[str, Any] = {"type": "step"}
+    for key in ("iteration", "dt", "CFL"):
+        rec[key] = diag.get(key)
+    rec["continuity"] = residuals.get("continuity")
+    rec["wall_time"] = diag.get("wall_time")
+    rec["residuals"] = {key: residuals.get(key) for key in ("Ru", "Rv", "Rp", "continuity")}
+    rec["diagnostics"] = {k: v for k, v in diag.items() if k not in _TOP_LEVEL_KEYS}
+    if seed is not None:
+        rec["run_meta"] = {"seed": seed}
+    return rec
+
+
+def _initial_state(ns: argparse.Namespace, cfg: SimulationConfig) -> tuple[State, int] | int:
+    """(state, first iteration), or the exit code of a refused restart."""
+    if not ns.restart:
+        return allocate_state(cfg.nx, cfg.ny), 0
+    from .io.checkpoint import load_checkpoint
+    try:
+        state, meta = load_checkpoint(ns.restart)
+    except Exception as exc:  # unreadable: start from rest, as the reference does
+        print(f"Failed to load checkpoint {ns.restart}: {exc}")
+        return allocate_state(cfg.nx, cfg.ny), 0
+    saved = meta.get("structured_config_hash")
+    if saved and saved != cfg.config_hash and not ns.allow_hash_mismatch:
+        print(f"Refusing restart: checkpoint hash {saved} != current {cfg.config_hash} "
+              "(use --allow-hash-mismatch to override)")
+        return EXIT_HASH_MISMATCH
+    try:
+        return state, int(meta.get("iteration", 0)) + 1
+    except (TypeError, ValueError) as exc:
+        print(f"Failed to load checkpoint {ns.restart}: {exc}")
+        return allocate_state(cfg.nx, cfg.ny), 0
+
+
+def _final_checkpoint(ns, cfg, last, quiet: bool) -> None:
+    """On Ctrl-C: write the last yielded state (best effort)."""
+    if not quiet:
+        print("KeyboardInterrupt received – attempting graceful checkpoint...")
+    if last is None:
+        return
+    state, diag = last
+    target = ns.checkpoint or "interrupt_final_checkpoint.npz"
+    try:
+        from .io.checkpoint import save_checkpoint
+        save_checkpoint(target, state, diag.get("iteration", 0), diag.get("wall_time", 0.0), cfg)
+    except Exception as exc:  # pragma: no cover - disk errors
+        if not quiet:
+            print(f"Failed to write final checkpoint: {exc}")
+        return
+    if not quiet:
+        print(f"Final checkpoint written: {target}")
+
+
+def _run(ns: argparse.Namespace, cfg: SimulationConfig) -> int:
+    if ns.precision:
+        os.environ["PYFLOW_PRECISION"] = ns.precision
+    start = _initial_state(ns, cfg)
+    if isinstance(start, int):
+        return start
+    state, first_it = start
+    quiet = bool(ns.json_stream)
+    if quiet:
+        cfg.force_quiet = True
+    if cfg.seed is not None:
+        import numpy as np
+        np.random.seed(int(cfg.seed))
+    driver = SimulationDriver(cfg, state, ResidualManager())
+    steps = driver.run(max_steps=ns.steps, start_iteration=first_it, progress=ns.progress and not quiet,
+                       checkpoint_path=ns.checkpoint, checkpoint_interval=_interval(ns))
+    threshold = ns.continuity_threshold
+    last = None
+    out = sys.stdout
+    try:
+        for st, residuals, diag in steps:
+            last = (st, diag)
+            if quiet:
+                out.write(_to_line(stream_record(residuals, diag, cfg.seed)) + "\n")
+                out.flush()
+            if threshold is None:
+                continue
+            cont = residuals.get("continuity", 1.0)
+            if cont < threshold:
+                if not quiet:
+                    print(f"Stopping early: continuity {cont:.3e} < threshold {threshold:.3e}")
+                break
+    except KeyboardInterrupt:
+        _final_checkpoint(ns, cfg, last, quiet)
+        return EXIT_INTERRUPTED
+    return EXIT_OK
+
+
+_to_line = json.JSONEncoder(separators=(",", ":")).encode
 
 
 def main(argv: list[str] | None = None) -> int:
-    parser = build_parser()
-    args = parser.parse_args(argv)
+    ns = build_parser().parse_args(argv)
     try:
-        ci = getattr(args, "checkpoint_interval", None)
-        ck_int = ci if ci not in (None, 0) else None
-        sim_cfg = SimulationConfig(
-            nx=args.nx, ny=args.ny, Re=args.re, lid_velocity=args.lid_velocity, cfl_target=args.cfl,
-            cfl_growth=args.cfl_growth, advection_scheme=args.scheme, disable_advection=args.disable_advection,
-            lin_tol=args.lin_tol, lin_maxiter=args.lin_maxiter, diagnostics=args.diagnostics,
-            checkpoint_interval=ck_int, log_path=args.log_jsonl, seed=args.seed, turbulence_model=None,
-        )
-    except (ValidationError, ConfigError) as e:
-        print("Configuration invalid:\n" + str(e))
-        return 2
-    if args.command == "validate":
-        out = sim_cfg.model_dump()
-        out["config_hash"] = sim_cfg.config_hash
-        print(json.dumps(out, indent=2, sort_keys=True))
-        return 0
-    if args.command == "show-config":
-        print(json.dumps({"hash": sim_cfg.config_hash, "brief": sim_cfg.brief()}, indent=2))
-        return 0
-    if args.precision:
-        os.environ["PYFLOW_PRECISION"] = args.precision
-    tracker = ResidualManager()
-    if args.restart:
-        try:
-            from .io.checkpoint import load_checkpoint
-            state, meta = load_checkpoint(args.restart)
-            ck_hash = meta.get("structured_config_hash")
-            if ck_hash and ck_hash != sim_cfg.config_hash and not args.allow_hash_mismatch:
-                print(f"Refusing restart: checkpoint hash {ck_hash} != current {sim_cfg.config_hash} "
-                      "(use --allow-hash-mismatch to override)")
-                return 3
-            start_it = int(meta.get("iteration", 0)) + 1
-        except Exception as e:
-            print(f"Failed to load checkpoint {args.restart}: {e}")
-            state = allocate_state(sim_cfg.nx, sim_cfg.ny)
-            start_it = 0
-    else:
-        state = allocate_state(sim_cfg.nx, sim_cfg.ny)
-        start_it = 0
-    driver = SimulationDriver(sim_cfg, state, tracker)
-    json_mode = bool(args.json_stream)
-    if json_mode:
-        sim_cfg.force_quiet = True
-    continuity_threshold = args.continuity_threshold
-    max_steps = args.steps
-    if sim_cfg.seed is not None:
-        import numpy as _np
-        _np.random.seed(int(sim_cfg.seed))
-    last_diag = None
-    last_state = None
-    try:
-        runtime_ck_int = None if args.checkpoint_interval in (None, 0) else args.checkpoint_interval
-        for st, residuals, diag in driver.run(max_steps=max_steps, start_iteration=start_it,
-                                              progress=(args.progress and not json_mode),
-                                              checkpoint_path=args.checkpoint, checkpoint_interval=runtime_ck_int):
-            last_diag = diag
-            last_state = st
-            if json_mode:
-                payload = {
-                    "type": "step",
-                    "iteration": diag.get("iteration"),
-                    "dt": diag.get("dt"),
-                    "CFL": diag.get("CFL"),
-                    "continuity": residuals.get("continuity"),
-                    "wall_time": diag.get("wall_time"),
-                    "residuals": {"Ru": residuals.get("Ru"), "Rv": residuals.get("Rv"),
-                                  "Rp": residuals.get("Rp"), "continuity": residuals.get("continuity")},
-                    "diagnostics": {k: v for k, v in diag.items() if k not in ("iteration", "dt", "CFL", "wall_time")},
-                }
-                if sim_cfg.seed is not None:
-                    payload.setdefault("run_meta", {})
-                    payload["run_meta"]["seed"] = sim_cfg.seed
-                sys.stdout.write(json.dumps(payload, separators=(",", ":")) + "\n")
-                sys.stdout.flush()
-            if continuity_threshold is not None and residuals.get("continuity", 1.0) < continuity_threshold:
-                if not json_mode:
-                    print(f"Stopping early: continuity {residuals['continuity']:.3e} < threshold {continuity_threshold:.3e}")
-                break
-    except KeyboardInterrupt:
-        if not json_mode:
-            print("KeyboardInterrupt received – attempting graceful checkpoint...")
-        ck_path = args.checkpoint or "interrupt_final_checkpoint.npz"
-        if last_state is not None and last_diag is not None:
-            try:
-                from .io.checkpoint import save_checkpoint
-                save_checkpoint(ck_path, last_state, last_diag.get("iteration", 0), last_diag.get("wall_time", 0.0), sim_cfg)
-                if not json_mode:
-                    print(f"Final checkpoint written: {ck_path}")
-            except Exception as e:  # pragma: no cover
-                if not json_mode:
-                    print(f"Failed to write final checkpoint: {e}")
-        return 130
-    return 0
+        cfg = _config(ns)
+    except (ValidationError, ConfigError) as exc:
+        print("Configuration invalid:\n" + str(exc))
+        return EXIT_BAD_CONFIG
+    if ns.command == "validate":
+        dump = cfg.model_dump()
+        dump["config_hash"] = cfg.config_hash
+        print(json.dumps(dump, indent=2, sort_keys=True))
+        return EXIT_OK
+    if ns.command == "show-config":
+        print(json.dumps({"hash": cfg.config_hash, "brief": cfg.brief()}, indent=2))
+        return EXIT_OK
+    return _run(ns, cfg)
 
 
 if __name__ == "__main__":  # pragma: no cover

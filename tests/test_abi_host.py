@@ -181,6 +181,38 @@ def test_golden_json_stream_schema():
                                          "divergence_norm", "nan_detected", "u_centerline"}
 
 
+def test_stream_record_rebuilds_reference_lines():
+    """``pyflow run --json-stream`` writes each step as the reference did:
+    rebuilding every line of the reference-written C1 stream from its own
+    residuals and diagnostics gives the same text, byte for byte."""
+    from pyflow.cli import _to_line, stream_record
+    with open(os.path.join(GOLDEN, "golden_cli_c1.jsonl")) as f:
+        texts = [l.rstrip("\n") for l in f if l.strip()]
+    assert len(texts) == 50
+    for text in texts:
+        line = json.loads(text)
+        diag = {"iteration": line["iteration"], "dt": line["dt"], "CFL": line["CFL"],
+                "wall_time": line["wall_time"], **line["diagnostics"]}
+        assert _to_line(stream_record(line["residuals"], diag)) == text
+    rec = stream_record({"Ru": 1.0}, {"iteration": 3}, seed=7)
+    assert rec["run_meta"] == {"seed": 7} and list(rec)[-1] == "run_meta"
+
+
+def test_jsonl_logger_file_append_and_close_all(tmp_path):
+    from pyflow.logging.structured import JsonlLogger, close_all_jsonl_logs
+    path = tmp_path / "sub" / "log.jsonl"
+    a = JsonlLogger(path=str(path))
+    a.log({"type": "step", "it": 0, "ts": 1.5})
+    b = JsonlLogger(path=str(path))
+    b.log({"type": "step", "it": 1})
+    close_all_jsonl_logs()
+    assert a._own_file.closed and b._own_file.closed
+    rows = [json.loads(l) for l in path.read_text().splitlines()]
+    assert rows[0] == {"type": "step", "it": 0, "ts": 1.5}
+    assert rows[1]["it"] == 1 and isinstance(rows[1]["ts"], float)
+    assert path.read_text().splitlines()[0] == '{"type":"step","it":0,"ts":1.5}'
+
+
 def test_residual_manager_and_state_api():
     from pyflow.core.ghost_fields import State, allocate_state, ghost_shapes, interior_view
     from pyflow.residuals.manager import ResidualManager
