@@ -87,8 +87,8 @@ typedef struct pf_cavity_params {
    * order of the FP64 partial sums, not the arithmetic of any cell. */
   int32_t tile_rows;       /* region height in 16-row tiles: 1, 2, 4 or 8 (regions are 64 cells wide) */
   int32_t max_workgroups;  /* cap on the cavity's workgroups (regions are grouped to fit) */
-  int32_t pcg_per_round;   /* MIXED, FP32 iterate: PCG iterations per round, 1 or 2 (s-step, s = 2) */
-  int32_t reserved;        /* 0 */
+  int32_t pcg_per_round;   /* MIXED: PCG iterations per round, 1 or 2 (s-step, s = 2) */
+  int32_t iterate64;       /* MIXED: PCG iterate in 1 = FP32, 2 = FP64; 0 = FP64 from 1536 cells a side */
 } pf_cavity_params;
 
 /* Per-cavity results of the last completed step / projection / solve. */
@@ -144,10 +144,24 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
 int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, size_t bytes, int* slot);
 int pf_batch_remove(pf_batch* b, int slot);
 
-/* Replace the parameters of cavity `cav` (same nx, ny). */
+/* Replace the parameters of cavity `cav` (same nx, ny). The tiling chosen when
+ * the cavity was added is kept. */
 int pf_batch_set_params(pf_batch* b, int cav, const pf_cavity_params* params);
 
-/* Host float64 <-> device fields of one cavity. Any of u, v, p may be NULL. */
+/* The parameters cavity `cav` runs with, every library default resolved:
+ * tile_rows = its region height, max_workgroups = its workgroup count,
+ * pcg_per_round (1 or 2) and iterate64 (1 or 2; 0 in FP64). Passing them back
+ * (pf_batch_add / pf_batch_create) reproduces the same tiling and so the same
+ * FP64 partial-sum order whatever PYFLOW_* environment the process has; the
+ * checkpoint meta written at io/checkpoint.py:66-76 carries them so that a
+ * restart continues bit for bit. */
+int pf_batch_resolved_params(pf_batch* b, int cav, pf_cavity_params* out);
+
+/* Host float64 <-> device fields of one cavity. Any of u, v, p may be NULL.
+ * Called between runs. MIXED fields are converted through an FP64 staging area
+ * inside the cavity's own workspace (the predictor buffers, which hold nothing
+ * between steps), so the right-hand side of pf_batch_set_rhs and the last
+ * solution stay intact. */
 int pf_batch_upload(pf_batch* b, int cav, const double* u, const double* v, const double* p,
                     int64_t ld, int ghost);
 int pf_batch_download(pf_batch* b, int cav, double* u, double* v, double* p,

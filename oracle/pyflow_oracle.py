@@ -351,6 +351,7 @@ def solve_pressure_poisson(u, v, p, dt, dx, dy, cfg, rhs_override=None,
         "divergence_norm_after_raw": div_after_raw,
         "divergence_norm_final": div_norm,
         "Rp_true_iterations": res.true_iterations,
+        "rhs_norm": float(np.linalg.norm(rhs)),
     }
     return psol, diag
 
@@ -465,6 +466,7 @@ def step(cfg: Any, state: OracleState, iteration: int = 0, ru_first: float = 1.0
         # extras (not in the reference's dict)
         "Rp_true_iterations": pdiag["Rp_true_iterations"],
         "divergence_norm_initial": pdiag["divergence_norm_initial"],
+        "Rp_rhs_norm": pdiag["rhs_norm"],
         "safeguard": safeguard,
         "N": N,
     }

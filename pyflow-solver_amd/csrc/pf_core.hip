@@ -71,6 +71,7 @@ struct pf_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<PfCav> hcav;       // slot -> description (nx == 0: free)
+  std::vector<pf_cavity_params> hpar;  // slot -> the caller's parameters
   std::vector<char> used;
   PfCav* dcav = nullptr;
   PfScal* dscal = nullptr;
@@ -122,6 +123,7 @@ static int check_params(const pf_cavity_params* p) {
     return fail(PF_EINVAL, "tile_rows must be 0, 1, 2, 4 or 8");
   if (p->max_workgroups < 0) return fail(PF_EINVAL, "max_workgroups must be >= 0");
   if (p->pcg_per_round < 0 || p->pcg_per_round > 2) return fail(PF_EINVAL, "pcg_per_round must be 0, 1 or 2");
+  if (p->iterate64 < 0 || p->iterate64 > 2) return fail(PF_EINVAL, "iterate64 must be 0, 1 or 2");
   return PF_OK;
 }
 
@@ -218,10 +220,12 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   // iterate to FP32 perturbs the true residual by about eps32 * cond(A) of the
   // pass's starting residual; cond(A) ~ (2n/pi)^2 makes that 0.1 at n = 2048
   // and 0.4 at n = 4096, where the refinement passes stop contracting.
-  // PYFLOW_XD=0/1 forces it off/on.
+  // A cavity's iterate64 (1: FP32, 2: FP64) decides; with 0, PYFLOW_XD=0/1
+  // forces it off/on.
   const int xd_env = env_switch("PYFLOW_XD", -1);
   const int nmax = std::max(p->nx, p->ny);
-  c.xd = (prec == PF_PREC_MIXED) ? (xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0)) : 0;
+  const int xd_dflt = xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0);
+  c.xd = (prec == PF_PREC_MIXED) ? (p->iterate64 > 0 ? (p->iterate64 == 2 ? 1 : 0) : xd_dflt) : 0;
   choose_ca(c, prec, p->pcg_per_round);
 }
 
@@ -466,16 +470,25 @@ static void fill_diag(const pf_batch* b, int slot, pf_step_diag* d) {
   d->outer_passes_run = s.outer_run;
 }
 
+// Host <-> device field copies between runs. FP64 fields are copied in
+// place; MIXED fields convert through an FP64 staging area, the cavity's
+// predictor pair u*, v* (contiguous, 2 x align256(4 n) >= 8 n bytes). The
+// predictor pair is written by PRED_A and dead once a step's CORR_B has run,
+// so between runs it holds nothing: staging through it leaves the Poisson
+// right-hand side b (pf_batch_set_rhs) and the solution untouched.
+static double* stage_of(PfCav& c) { return (double*)c.us; }
+
 template <class T>
 static int upload_field(pf_batch* b, PfCav& c, T* dst, const double* src, int64_t ld, int ghost) {
   const size_t n = (size_t)c.nx * c.ny;
   const double* s0 = src + (ghost ? (ld + 1) : 0);
-  double* stage = c.b;  // b is recomputed every step; free between runs
-  HIPCHK(hipMemcpy2DAsync(stage, sizeof(double) * c.nx, s0, sizeof(double) * ld, sizeof(double) * c.nx, c.ny,
-                          hipMemcpyHostToDevice, b->stream));
   if (sizeof(T) == sizeof(double)) {
-    HIPCHK(hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(double) * c.nx, s0, sizeof(double) * ld, sizeof(double) * c.nx, c.ny,
+                            hipMemcpyHostToDevice, b->stream));
   } else {
+    double* stage = stage_of(c);
+    HIPCHK(hipMemcpy2DAsync(stage, sizeof(double) * c.nx, s0, sizeof(double) * ld, sizeof(double) * c.nx, c.ny,
+                            hipMemcpyHostToDevice, b->stream));
     k_f64_to<T><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream>>>(stage, dst, n);
   }
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -486,13 +499,13 @@ template <class T>
 static int download_field(pf_batch* b, PfCav& c, const T* src, double* dst, int64_t ld, int ghost) {
   const size_t n = (size_t)c.nx * c.ny;
   double* d0 = dst + (ghost ? (ld + 1) : 0);
-  double* stage = c.b;
-  if (sizeof(T) == sizeof(double)) {
-    stage = (double*)src;
-  } else {
+  const double* from = (const double*)src;
+  if (sizeof(T) != sizeof(double)) {
+    double* stage = stage_of(c);
     k_to_f64<T><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream>>>(src, stage, n);
+    from = stage;
   }
-  HIPCHK(hipMemcpy2DAsync(d0, sizeof(double) * ld, stage, sizeof(double) * c.nx, sizeof(double) * c.nx, c.ny,
+  HIPCHK(hipMemcpy2DAsync(d0, sizeof(double) * ld, from, sizeof(double) * c.nx, sizeof(double) * c.nx, c.ny,
                           hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PF_OK;
@@ -557,6 +570,7 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->max_tiles = max_tiles;
   b->stream = (hipStream_t)stream;
   b->hcav.assign(max_cav, PfCav{});
+  b->hpar.assign(max_cav, pf_cavity_params{});
   b->unroll = env_int("PYFLOW_UNROLL", 2);
   b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 64));
   b->used.assign(max_cav, 0);
@@ -630,6 +644,7 @@ int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, s
   c.part = (double*)w;
   HIPCHK(hipMemsetAsync(workspace, 0, need, b->stream));
   b->hcav[s] = c;
+  b->hpar[s] = *params;
   b->used[s] = 1;
   PfScal& sc = b->hscal[s];
   std::memset(&sc, 0, sizeof(sc));
@@ -647,6 +662,7 @@ int pf_batch_remove(pf_batch* b, int slot) {
   if (rc) return rc;
   b->used[slot] = 0;
   b->hcav[slot] = PfCav{};
+  b->hpar[slot] = pf_cavity_params{};
   std::memset(&b->hscal[slot], 0, sizeof(PfScal));
   b->tables_dirty = true;
   return push_scal(b, slot);
@@ -720,7 +736,22 @@ int pf_batch_set_params(pf_batch* b, int slot, const pf_cavity_params* params) {
   c.nreg = nreg;
   c.rpw = rpw;
   choose_ca(c, b->prec, params->pcg_per_round);
+  b->hpar[slot] = *params;
   b->tables_dirty = true;
+  return PF_OK;
+}
+
+int pf_batch_resolved_params(pf_batch* b, int slot, pf_cavity_params* out) {
+  if (!out) return fail(PF_EINVAL, "out is NULL");
+  int rc = check_slot(b, slot);
+  if (rc) return rc;
+  const PfCav& c = b->hcav[slot];
+  *out = b->hpar[slot];
+  out->tile_rows = c.tpb;
+  out->max_workgroups = c.nblk;
+  out->pcg_per_round = (b->prec == PF_PREC_MIXED) ? (c.ca ? 2 : 1) : 1;
+  out->iterate64 = (b->prec == PF_PREC_MIXED) ? (c.xd ? 2 : 1) : 0;
+  out->inner_rtol = c.inner_rtol;
   return PF_OK;
 }
 

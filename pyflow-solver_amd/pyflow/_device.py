@@ -144,6 +144,13 @@ class DeviceBatch:
     def params(self, slot: int) -> H.CavityParams:
         return self.slots[slot]["params"]
 
+    def resolved_params(self, slot: int) -> H.CavityParams:
+        """The slot's parameters with the library's defaults resolved (tiling,
+        iterations per round, FP64 iterate, inner tolerance)."""
+        out = H.CavityParams()
+        H.check(self.L.pf_batch_resolved_params(self.handle, int(slot), C.byref(out)), "pf_batch_resolved_params")
+        return out
+
     # -- fields -----------------------------------------------------------
     def upload(self, slot: int, u=None, v=None, p=None, ghost: bool = True) -> None:
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, p)]
@@ -280,6 +287,9 @@ class DeviceCavity:
 
     def set_params(self, p: H.CavityParams) -> None:
         self.batch.set_params(self.slot, p)
+
+    def resolved_params(self) -> H.CavityParams:
+        return self.batch.resolved_params(self.slot)
 
     def upload(self, u=None, v=None, p=None, ghost=True):
         self.batch.upload(self.slot, u, v, p, ghost)

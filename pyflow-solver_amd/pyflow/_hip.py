@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "pf_batch_set_dt", "pf_batch_run", "pf_batch_launch", "pf_batch_poll", "pf_batch_diag",
     "pf_batch_centerline", "pf_batch_rounds", "pf_batch_set_timing", "pf_batch_timing", "pf_op_apply",
     "pf_batch_start", "pf_batch_advance", "pf_batch_finished", "pf_batch_set_graph",
+    "pf_batch_resolved_params",
 )
 
 
@@ -55,7 +56,7 @@ class CavityParams(C.Structure):
         ("cfl_target", C.c_double), ("cfl_growth", C.c_double), ("lin_tol", C.c_double),
         ("inv_diag", C.c_double), ("inner_rtol", C.c_double),
         ("tile_rows", C.c_int32), ("max_workgroups", C.c_int32), ("pcg_per_round", C.c_int32),
-        ("reserved", C.c_int32),
+        ("iterate64", C.c_int32),
     ]
 
 
@@ -102,6 +103,7 @@ def _declare(L):
         "pf_batch_add": ([P, C.POINTER(CavityParams), P, Z, C.POINTER(I)], I),
         "pf_batch_remove": ([P, I], I),
         "pf_batch_set_params": ([P, I, C.POINTER(CavityParams)], I),
+        "pf_batch_resolved_params": ([P, I, C.POINTER(CavityParams)], I),
         "pf_batch_upload": ([P, I, P, P, P, I64, I], I),
         "pf_batch_download": ([P, I, P, P, P, I64, I], I),
         "pf_batch_zero_fields": ([P], I),

@@ -1,4 +1,14 @@
-"""Device mirror of a ``State``: one device cavity plus the host/device sync rules."""
+"""Device mirror of a ``State``: one device cavity plus the host/device sync rules.
+
+The arithmetic of a cavity is fixed by its precision and by the tiling the
+library resolved when the cavity was created (region height, workgroups,
+iterations per round, FP64 iterate, inner tolerance): the tiling sets the
+order of the FP64 partial sums. Both are recorded in
+``state.meta['kernel_config']`` the first time a state reaches the device, are
+written into checkpoints, and are applied again to every later step of that
+state, so a resumed run repeats the uninterrupted one bit for bit whatever
+``PYFLOW_*`` environment or tuner pick the new process has.
+"""
 from __future__ import annotations
 
 import weakref
@@ -13,8 +23,43 @@ __all__ = ["StateMirror", "mirror_for", "config_precision"]
 _DEVICE_FIELDS = ("u", "v", "p")
 
 
-def config_precision(cfg) -> int:
-    return precision_code(getattr(cfg, "precision", None))
+_PINNED_INT = ("tile_rows", "max_workgroups", "pcg_per_round", "iterate64")
+
+
+def _precision_name(prec: int) -> str:
+    return "mixed" if prec == H.PREC_MIXED else "fp64"
+
+
+def config_precision(cfg, state=None) -> int:
+    """The config's ``precision`` if it sets one, else the precision recorded
+    with the state (a resumed checkpoint), else ``PYFLOW_PRECISION``."""
+    explicit = getattr(cfg, "precision", None)
+    if explicit is None and state is not None:
+        explicit = (state.meta.get("kernel_config") or {}).get("precision")
+    return precision_code(explicit)
+
+
+def _pin(state, params: H.CavityParams, prec: int) -> None:
+    """Apply the state's recorded kernel config to ``params`` (in place)."""
+    kc = state.meta.get("kernel_config")
+    if not kc or kc.get("precision") != _precision_name(prec):
+        return  # nothing recorded, or another precision was asked for explicitly
+    if (int(kc["nx"]), int(kc["ny"])) != (params.nx, params.ny):
+        raise ValueError(f"state.meta['kernel_config'] was recorded for a {kc['nx']}x{kc['ny']} grid, "
+                         f"not {params.nx}x{params.ny}")
+    for k in _PINNED_INT:
+        setattr(params, k, int(kc[k]))
+    params.inner_rtol = float(kc["inner_rtol"])
+
+
+def _record(state, mirror: "StateMirror") -> None:
+    r = mirror.cav.resolved_params()
+    kc = {"precision": _precision_name(mirror.prec), "nx": int(r.nx), "ny": int(r.ny),
+          **{k: int(getattr(r, k)) for k in _PINNED_INT}, "inner_rtol": float(r.inner_rtol)}
+    old = state.meta.get("kernel_config")
+    if old and old.get("precision") == kc["precision"] and any(old[k] != kc[k] for k in kc):
+        raise RuntimeError(f"the device cavity runs with {kc}, not the recorded kernel config {old}")
+    state.meta["kernel_config"] = kc
 
 
 class StateMirror:
@@ -83,6 +128,9 @@ class StateMirror:
 
 
 def mirror_for(state, params: H.CavityParams, prec: int) -> StateMirror:
+    """The state's device cavity for ``params``; ``params`` gets the state's
+    recorded kernel config (pass the same object to ``before``)."""
+    _pin(state, params, prec)
     m = state._dev
     if m is not None and (m.nx, m.ny, m.prec) == (params.nx, params.ny, prec):
         return m
@@ -92,4 +140,5 @@ def mirror_for(state, params: H.CavityParams, prec: int) -> StateMirror:
         state._dev = None
     m = StateMirror(state, params, prec)
     state._dev = m
+    _record(state, m)
     return m

@@ -4,9 +4,16 @@ Same ``.npz`` format: key ``meta`` (JSON string) and ``field_<name>`` arrays
 with the ghost ring, atomic write (temp file, fsync, rename), SHA1 field
 hashes (``sha1(arr.tobytes())[:16]``), ``config_hash`` and
 ``structured_config_hash``. Added meta keys (ignored by the reference's
-loader): ``dt_prev`` and ``divergence_history`` -- the dt controller's memory
-(state.meta), which the reference does not persist -- and ``precision``; with
-them a restart continues the uninterrupted trajectory bit for bit.
+loader), all taken from and restored into ``state.meta``:
+
+* ``dt_prev`` and ``divergence_history``: the dt controller's memory, which
+  the reference does not persist;
+* ``kernel_config``: the precision and the tiling the device cavity ran with
+  (region height, workgroups, PCG iterations per round, FP64 iterate, inner
+  tolerance). The tiling fixes the order of the FP64 partial sums; the next
+  step of a loaded state runs with it again, whatever the environment.
+
+With them a restart continues the uninterrupted trajectory bit for bit.
 """
 from __future__ import annotations
 
@@ -28,7 +35,7 @@ except Exception:  # pragma: no cover
 
 __all__ = ["save_checkpoint", "load_checkpoint", "save_checkpoint_bytes", "load_checkpoint_bytes"]
 
-_RESTART_META = ("dt_prev", "divergence_history")
+_RESTART_META = ("dt_prev", "divergence_history", "kernel_config")
 
 
 def _hash_config(cfg: Any) -> str:
@@ -54,13 +61,12 @@ def _fields(state: State) -> dict[str, np.ndarray]:
 
 def _restart_meta(state: State, cfg: Any) -> dict[str, Any]:
     out: dict[str, Any] = {}
-    for k in _RESTART_META:
-        if k in state.meta:
-            v = state.meta[k]
-            out[k] = [float(x) for x in v] if isinstance(v, (list, tuple)) else float(v)
-    dev = getattr(state, "_dev", None)
-    if dev is not None:
-        out["precision"] = "mixed" if dev.prec == 1 else "fp64"
+    if "dt_prev" in state.meta:
+        out["dt_prev"] = float(state.meta["dt_prev"])
+    if "divergence_history" in state.meta:
+        out["divergence_history"] = [float(x) for x in state.meta["divergence_history"]]
+    if state.meta.get("kernel_config"):
+        out["kernel_config"] = dict(state.meta["kernel_config"])
     return out
 
 
@@ -114,7 +120,7 @@ def _state_from(data) -> tuple[State, dict[str, Any]]:
     for k in _RESTART_META:
         if k in meta:
             v = meta[k]
-            st.meta[k] = list(v) if isinstance(v, list) else v
+            st.meta[k] = list(v) if isinstance(v, list) else (dict(v) if isinstance(v, dict) else v)
     return st, meta
 
 

@@ -77,7 +77,7 @@ def solve_pressure_poisson(state: State, dt: float, dx: float, dy: float, cfg, p
                                jacobi=_jacobi_flag(preconditioner, cfg, nx, ny, dx, dy, True),
                                op=H.OP_NEG_LAP_PINNED, mode=H.MODE_PROJECT,
                                inner_rtol=float(getattr(cfg, "inner_rtol", D.DEFAULT_INNER_RTOL)))
-    mirror = mirror_for(state, params, config_precision(cfg))
+    mirror = mirror_for(state, params, config_precision(cfg, state))
     mirror.before(params)
     if manufactured:
         rhs = np.copy(np.asarray(rhs_override, dtype=np.float64)).reshape(-1)

@@ -59,7 +59,7 @@ def step(config, state: State, tracker, iteration: int) -> tuple[State, dict[str
     Re = float(getattr(config, "Re", 100.0))
     nu = 1.0 / Re
     params = step_params(config, nx, ny, dx, dy)
-    mirror = mirror_for(state, params, config_precision(config))
+    mirror = mirror_for(state, params, config_precision(config, state))
     mirror.before(params)
     hist = state.meta.get("divergence_history", [])
     mirror.cav.set_meta(float(state.meta.get("dt_prev", 1e-2)), list(hist)[-5:])

@@ -277,22 +277,78 @@ def test_autotune_candidates_are_distinct_and_valid():
                 assert prec == "mixed" and A.blocking(n, n, c.tile_rows, c.max_workgroups)[0] <= 4
 
 
-def test_serving_footprint_and_first_fit():
-    """A request's footprint is its cavity workspace plus allocation slack;
-    admission takes waiting requests in arrival order when they fit."""
+def test_serving_footprint_and_arena():
+    """A request's footprint is its cavity workspace (a multiple of 256 B);
+    the arena places footprints first fit, merges freed neighbours and never
+    hands out overlapping blocks."""
     import ctypes
     import pyflow._hip as H
-    from pyflow.serving import CavityRequest, first_fit, footprint
+    from pyflow.serving import Arena, CavityRequest, footprint
     L = H.lib()
     for nx, ny in ((64, 64), (130, 97), (2048, 2048)):
         req = CavityRequest(nx, ny, steps=10, Re=400.0)
         p = req.params()
         n = ctypes.c_size_t()
         assert L.pf_batch_workspace_bytes(H.PREC_MIXED, 1, ctypes.byref(p), ctypes.byref(n)) == 0
-        assert footprint(req, "mixed") == n.value + 256
+        assert n.value % 256 == 0 and footprint(req, "mixed") == n.value
         assert footprint(req, "fp64") > footprint(req, "mixed")
-    assert first_fit([5, 3, 4, 1], free_bytes=8, free_slots=4) == [0, 1]
-    assert first_fit([9, 3, 4, 1], free_bytes=8, free_slots=4) == [1, 2, 3]
-    assert first_fit([1, 1, 1], free_bytes=8, free_slots=2) == [0, 1]
-    assert first_fit([], free_bytes=8, free_slots=2) == []
-    assert first_fit([9], free_bytes=8, free_slots=2) == []
+    a = Arena(10 * 256 + 100)
+    assert a.size == 2560 and a.free_bytes == 2560
+    x = a.alloc(300)   # 512 B at 0
+    y = a.alloc(256)   # at 512
+    z = a.alloc(1024)  # at 768
+    assert (x, y, z) == (0, 512, 768) and a.free_bytes == 768
+    assert a.alloc(1024) is None and a.largest_free == 768
+    a.release(y, 256)
+    assert a.alloc(512) == 1792  # the freed 256-byte hole is too small: first fit goes on
+    a.release(x, 300)
+    assert a.largest_free == 768  # 0..512 merged with 512..768
+    assert a.alloc(700) == 0
+    with pytest.raises(ValueError):
+        a.release(768, 1024)
+        a.release(768, 1024)
+    rng = np.random.default_rng(0)
+    b = Arena(1 << 20)
+    live = {}
+    for _ in range(2000):
+        if live and rng.random() < 0.45:
+            off = list(live)[int(rng.integers(len(live)))]
+            b.release(off, live.pop(off))
+        else:
+            n = int(rng.integers(1, 40000))
+            off = b.alloc(n)
+            if off is not None:
+                live[off] = n
+        spans = sorted((o, o + -(-n // 256) * 256) for o, n in live.items())
+        assert all(e1 <= s2 for (_, e1), (s2, _) in zip(spans, spans[1:]))
+        assert all(e <= b.size for _, e in spans)
+        assert b.free_bytes + sum(e - s for s, e in spans) == b.size
+
+
+def test_checkpoint_round_trips_restart_meta(tmp_path):
+    """dt controller memory and the kernel config survive save/load, and a
+    recorded config is applied to the step parameters (CPU: no device call)."""
+    from pyflow._device import cavity_params
+    from pyflow._mirror import _pin, config_precision
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.io.checkpoint import load_checkpoint, save_checkpoint
+    import pyflow._hip as H
+    st = allocate_state(20, 18)
+    kc = {"precision": "mixed", "nx": 20, "ny": 18, "tile_rows": 2, "max_workgroups": 3, "pcg_per_round": 1,
+          "iterate64": 2, "inner_rtol": 0.03}
+    st.meta.update(dt_prev=0.25, divergence_history=[1.0, 2.0, 3.5], kernel_config=kc)
+    ck = str(tmp_path / "ck.npz")
+    save_checkpoint(ck, st, 7, 0.0, SimpleNamespace(seed=None))
+    loaded, meta = load_checkpoint(ck)
+    assert loaded.meta["dt_prev"] == 0.25 and loaded.meta["divergence_history"] == [1.0, 2.0, 3.5]
+    assert loaded.meta["kernel_config"] == kc and meta["iteration"] == 7
+    assert config_precision(SimpleNamespace(), loaded) == H.PREC_MIXED
+    assert config_precision(SimpleNamespace(precision="fp64"), loaded) == H.PREC_FP64
+    p = cavity_params(20, 18, dx=1.0, dy=1.0)
+    _pin(loaded, p, H.PREC_MIXED)
+    assert (p.tile_rows, p.max_workgroups, p.pcg_per_round, p.iterate64, p.inner_rtol) == (2, 3, 1, 2, 0.03)
+    q = cavity_params(20, 18, dx=1.0, dy=1.0)
+    _pin(loaded, q, H.PREC_FP64)  # another precision asked for explicitly: nothing pinned
+    assert (q.tile_rows, q.max_workgroups, q.pcg_per_round, q.iterate64) == (0, 0, 0, 0)
+    with pytest.raises(ValueError):
+        _pin(loaded, cavity_params(21, 18, dx=1.0, dy=1.0), H.PREC_MIXED)

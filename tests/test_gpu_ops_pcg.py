@@ -195,3 +195,42 @@ def test_device_cg_two_iterations_per_round(n, monkeypatch):
     op, b = cases[0][1], cases[0][4]
     x, info, it, rnorm, conv = device_cg(op, b, 1e-14, 7, jacobi=True, precision="mixed")
     assert info == 7 and it == 7 and not conv
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_field_copies_leave_rhs_and_solution_intact(precision):
+    """Uploading or downloading fields between runs does not touch the
+    Poisson right-hand side or the last solution (the copies stage through
+    the cavity's predictor buffers): set_rhs -> upload -> run gives the same
+    solution as set_rhs -> run, and a download after the run leaves
+    get_solution unchanged."""
+    from pyflow import _hip as H
+    from pyflow._device import DeviceBatch, cavity_params
+    nx, ny = 70, 45
+    rhs = np.random.default_rng(5).standard_normal((ny, nx))
+    rhs -= rhs.mean()
+    rhs[0, 0] = 0.0
+    p = cavity_params(nx, ny, dx=1.0, dy=1.0, lin_tol=1e-9, lin_maxiter=5000, jacobi=True,
+                      op=H.OP_NEG_LAP_PINNED, mode=H.MODE_SOLVE)
+    ref = DeviceBatch([p], precision)
+    ref.set_rhs(0, rhs)
+    d_ref = ref.run(1)[0]
+    x_ref = ref.solution(0)
+    ref.close()
+    assert d_ref.rp_converged
+    b = DeviceBatch([p], precision)
+    b.set_rhs(0, rhs)
+    u = np.random.default_rng(6).standard_normal((ny + 2, nx + 2))
+    b.upload(0, u, u, u)
+    d = b.run(1)[0]
+    x1 = b.solution(0)
+    got = np.zeros((ny + 2, nx + 2))
+    b.download(0, got, got.copy(), got.copy())
+    x2 = b.solution(0)
+    b.close()
+    assert np.array_equal(x1, x_ref) and np.array_equal(x2, x_ref)
+    assert d.rp_iterations == d_ref.rp_iterations and d.rp_residual == d_ref.rp_residual
+    if precision == "fp64":
+        assert np.array_equal(got[1:-1, 1:-1], u[1:-1, 1:-1])
+    else:
+        assert np.array_equal(got[1:-1, 1:-1], u[1:-1, 1:-1].astype(np.float32).astype(np.float64))

@@ -15,12 +15,19 @@ pytestmark = pytest.mark.gpu
 # FP64 parity bound. The stencils are bit-identical to the reference; only the
 # reduction order of dot products, norms and the RHS mean differs, and CG
 # amplifies last-bit differences. tests/golden/golden_envelope.json holds, per
-# golden case and quantity, how far the reference trajectory itself moves when
-# only its summation order changes (exactly rounded sums). The GPU must stay
-# within ENVELOPE_FACTOR times that, and u, v within the 1e-5 relative field
-# bound -- or within twice the case's own envelope where the reference itself
-# moves by more than 5e-6 under a different summation order.
-ENVELOPE_FACTOR = 50.0
+# golden case and quantity, the largest change of the reference trajectory
+# itself over six other summation orders (exactly rounded, running sums
+# forwards and backwards, 64-element blocks, two random orders). The GPU must
+# stay within ENVELOPE_FACTOR times that, and u, v also within the 1e-5
+# relative field bound (or twice the case's own envelope where the reference
+# itself moves by more than 5e-6).
+# Rp_residual, the true residual |b - A x| at the end of the solve, is set by
+# where CG stops: for a converged solve it lies anywhere below lin_tol*|b|
+# (its envelope is 30%-140% relative), so there both the reference's and the
+# GPU's values must lie in [0, lin_tol*|b|] -- their difference is bounded by
+# the stopping threshold, not by a relative envelope. A solve that ran to
+# lin_maxiter is held to the relative envelope like every other quantity.
+ENVELOPE_FACTOR = 4.0
 FIELD_UV_CAP = 1e-5
 FLOOR_REL = 1e-12
 
@@ -69,6 +76,7 @@ def test_step_trajectory_matches_reference_fp64(golden_steps, name):
     st, rows = _run_case(golden_steps, name)
     D = s[f"st_{name}_diag"]
     worst = {}
+    rp_abs = 0.0  # largest |delta Rp_residual| / (lin_tol |b|) over converged steps
     for it, row in enumerate(rows):
         exp = dict(zip(keys, D[it]))
         assert row["Rp_iterations"] == int(exp["Rp_iterations"]), (it, row["Rp_iterations"], exp["Rp_iterations"])
@@ -76,11 +84,19 @@ def test_step_trajectory_matches_reference_fp64(golden_steps, name):
         assert row["nan_detected"] == bool(exp["nan_detected"])
         for k in STEP_KEYS_FLOAT:
             e = exp[k]
+            if k == "Rp_residual" and bool(exp["Rp_converged"]):
+                bound = env["rp_abs_bound"][it]
+                assert 0.0 <= row[k] <= bound * (1 + 1e-12) and e <= bound * (1 + 1e-12), (name, it, row[k], e, bound)
+                if bound > 0:
+                    rp_abs = max(rp_abs, abs(row[k] - e) / bound)
+                continue
             rel = abs(row[k] - e) / abs(e) if e != 0 else abs(row[k] - e)
             worst[k] = max(worst.get(k, 0.0), rel)
+    print(f"[step] {name} Rp_residual (converged steps): |delta| / (lin_tol |b|) {rp_abs:.2e} (bound 1)", flush=True)
     for k, w in worst.items():
         tol = max(FLOOR_REL, ENVELOPE_FACTOR * env[k])
-        print(f"[step] {name} {k}: rel err {w:.2e} (envelope {env[k]:.2e})", flush=True)
+        print(f"[step] {name} {k}: rel err {w:.2e} (envelope {env[k]:.2e}, ratio {w / max(env[k], FLOOR_REL):.2f})",
+              flush=True)
         assert w <= tol, (name, k, w, env[k])
     for f in ("u", "v", "p"):
         ref = s[f"st_{name}_{f}"][-1]
@@ -89,8 +105,10 @@ def test_step_trajectory_matches_reference_fp64(golden_steps, name):
         assert np.array_equal(got[0, :], ref[0, :]) and np.array_equal(got[:, 0], ref[:, 0])  # ghost ring
         maxrel = _rel(got, ref)
         l2rel = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
-        print(f"[step] {name} field {f}: max rel {maxrel:.2e} (envelope {env[f'field_{f}_maxrel']:.2e}), "
-              f"l2 rel {l2rel:.2e} (envelope {env[f'field_{f}_l2rel']:.2e})", flush=True)
+        print(f"[step] {name} field {f}: max rel {maxrel:.2e} (envelope {env[f'field_{f}_maxrel']:.2e}, ratio "
+              f"{maxrel / max(env[f'field_{f}_maxrel'], FLOOR_REL):.2f}), l2 rel {l2rel:.2e} (envelope "
+              f"{env[f'field_{f}_l2rel']:.2e}, ratio {l2rel / max(env[f'field_{f}_l2rel'], FLOOR_REL):.2f})",
+              flush=True)
         assert maxrel <= max(FLOOR_REL, ENVELOPE_FACTOR * env[f"field_{f}_maxrel"]), (name, f, maxrel)
         assert l2rel <= max(FLOOR_REL, ENVELOPE_FACTOR * env[f"field_{f}_l2rel"]), (name, f, l2rel)
         if f in ("u", "v"):
@@ -239,6 +257,55 @@ def test_restart_is_bitwise(tmp_path):
         for f in ("u", "v", "p"):
             assert np.array_equal(loaded.fields[f], full.fields[f]), (precision, f)
         assert loaded.meta["dt_prev"] == full.meta["dt_prev"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_restart_is_bitwise_under_another_environment(tmp_path, monkeypatch, precision):
+    """The checkpoint pins the tiling: resumed in a process whose PYFLOW_*
+    settings pick another region height, workgroup count and (MIXED) one PCG
+    iteration per round, the run still repeats the uninterrupted one bit for
+    bit. Without the pinned config the same resume gives other bits."""
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.drivers.simulation_driver import SimulationDriver
+    from pyflow.io.checkpoint import load_checkpoint, save_checkpoint
+    from pyflow.residuals.manager import ResidualManager
+    nx, ny = 200, 150
+    cfg = SimpleNamespace(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05,
+                          Re=100.0, lid_velocity=1.0, lin_tol=1e-9, lin_maxiter=2000, nx=nx, ny=ny,
+                          diagnostics=False, precision=precision)
+    for k in ("PYFLOW_TPB", "PYFLOW_MAXWG", "PYFLOW_CA", "PYFLOW_XD"):
+        monkeypatch.delenv(k, raising=False)
+    full = allocate_state(nx, ny)
+    for _ in SimulationDriver(cfg, full, ResidualManager()).run(max_steps=6):
+        pass
+    part = allocate_state(nx, ny)
+    for _ in SimulationDriver(cfg, part, ResidualManager()).run(max_steps=3):
+        pass
+    kc = dict(part.meta["kernel_config"])
+    assert kc["precision"] == precision and kc["tile_rows"] == 4 and kc["max_workgroups"] == 12
+    ck = str(tmp_path / "ck.npz")
+    save_checkpoint(ck, part, 2, 0.0, cfg)
+    monkeypatch.setenv("PYFLOW_TPB", "1")
+    monkeypatch.setenv("PYFLOW_MAXWG", "7")
+    monkeypatch.setenv("PYFLOW_CA", "0")
+    monkeypatch.setenv("PYFLOW_XD", "1")
+    loaded, meta = load_checkpoint(ck)
+    assert meta["kernel_config"] == kc and loaded.meta["kernel_config"] == kc
+    for _ in SimulationDriver(cfg, loaded, ResidualManager()).run(max_steps=6, start_iteration=3):
+        pass
+    assert loaded.meta["kernel_config"] == kc
+    for f in ("u", "v", "p"):
+        assert np.array_equal(loaded.fields[f], full.fields[f]), (precision, f)
+    assert loaded.meta["dt_prev"] == full.meta["dt_prev"]
+    unpinned, _ = load_checkpoint(ck)
+    del unpinned.meta["kernel_config"]
+    for _ in SimulationDriver(cfg, unpinned, ResidualManager()).run(max_steps=6, start_iteration=3):
+        pass
+    assert unpinned.meta["kernel_config"]["tile_rows"] == 1
+    same = all(np.array_equal(unpinned.fields[f], full.fields[f]) for f in ("u", "v", "p"))
+    print(f"[restart] {precision}: unpinned resume bitwise equal = {same}", flush=True)
+    if precision == "mixed":
+        assert not same
 
 
 def test_preconditioner_effect_and_manufactured_projection():
