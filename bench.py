@@ -40,9 +40,12 @@ def parse():
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the float64 comparison runs")
-    ap.add_argument("--oracle-steps", type=int, default=0,
-                    help="steps from rest also compared with the FP64 NumPy oracle on the host (2 steps at "
-                         "1024x1024 take minutes of CPU time; default 0: skip)")
+    ap.add_argument("--oracle-steps", type=int, default=2,
+                    help="steps from rest also compared with the FP64 NumPy oracle on the host, outside the "
+                         "timed region (default 2: step 0 from rest has a zero right-hand side, step 1 is the "
+                         "first full projection; about 90 s of host time at 1024x1024; 0: skip)")
+    ap.add_argument("--no-drop-in", action="store_true",
+                    help="skip timing the drop-in path (SimulationDriver.run) and the C1 CPU baselines")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 GPU form")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -53,8 +56,10 @@ def parse():
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a plain 256 MiB device copy (10x) to calibrate PMC byte counters")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
-    ap.add_argument("--step-scale", type=float, default=0.02,
-                    help="serve: scale of the per-request step count U{100..2000} (default 0.02: U{2..40})")
+    ap.add_argument("--step-scale", type=float, default=1.0,
+                    help="serve: scale of the per-request step count U{100..2000} (default 1: the configuration)")
+    ap.add_argument("--hbm-budget-gib", type=float, default=4.0,
+                    help="serve: device memory the admitted requests may hold together (GiB)")
     ap.add_argument("--workload", choices=["cavity", "poisson", "batch", "serve"], default="cavity",
                     help="cavity: one lid-driven cavity (default, the headline metric); poisson: one 4096x4096 "
                          "pinned-Neumann solve to 1e-10 (FP32 PCG + FP64 refinement); batch: 256 cavities of "
@@ -139,6 +144,17 @@ def measured_traffic(n, precision, tile=None):
     return traffic_line(path)
 
 
+def _grad_rel(p, p_ref):
+    """Relative L2 difference of the face gradients of two ghost-layout
+    pressure fields (dx = dy = 1): the part of p the velocity sees. A constant
+    offset of p (the slowest mode of the pinned operator) does not enter."""
+    import numpy as np
+    a, b = p[1:-1, 1:-1], p_ref[1:-1, 1:-1]
+    num = np.sum(np.diff(a - b, axis=0) ** 2) + np.sum(np.diff(a - b, axis=1) ** 2)
+    den = np.sum(np.diff(b, axis=0) ** 2) + np.sum(np.diff(b, axis=1) ** 2)
+    return float(np.sqrt(num / max(den, 1e-300)))
+
+
 def oracle_accuracy(n, params, args):
     """The first steps of the run (from rest) on the GPU in mixed precision
     and with the FP64 NumPy oracle on the same inputs: field and continuity
@@ -169,6 +185,7 @@ def oracle_accuracy(n, params, args):
         "max_abs_du_over_U": float(np.max(np.abs(u - ref.fields["u"]))),
         "max_abs_dv_over_U": float(np.max(np.abs(v - ref.fields["v"]))),
         "rel_l2_dp": float(np.linalg.norm(p - po) / max(np.linalg.norm(po), 1e-300)),
+        "rel_l2_dgradp": _grad_rel(p, po),
         "continuity": float(d.continuity),
         "continuity_oracle": cont_o,
         "continuity_diff": float(abs(d.continuity - cont_o)),
@@ -176,18 +193,20 @@ def oracle_accuracy(n, params, args):
     }
 
 
-def cpu_baseline(n, iters_per_step, args):
-    """Time the FP64 NumPy oracle on this host: one step at n x n with the PCG
-    capped at m1 and at m2 iterations; t(m) = a + b m gives the non-PCG cost a
-    and the per-iteration cost b, extrapolated to the GPU's mean iterations."""
-    import numpy as np
+def _oracle_step_costs(n, threads):
+    """FP64 NumPy oracle: one n x n QUICK step (step 1 from rest) with the
+    PCG capped at 8 and at 40 iterations; t(m) = a + b m gives the cost a
+    outside the PCG and the cost b per iteration. ``threads``: BLAS threads
+    (None: the library's default, every core)."""
     from types import SimpleNamespace
     from oracle import pyflow_oracle as O
-    try:
-        from threadpoolctl import threadpool_limits
-        limiter = threadpool_limits(limits=1)
-    except Exception:  # pragma: no cover
-        limiter = None
+    limiter = None
+    if threads is not None:
+        try:
+            from threadpoolctl import threadpool_limits
+            limiter = threadpool_limits(limits=threads)
+        except Exception:  # pragma: no cover
+            limiter = None
     cfg0 = dict(disable_advection=False, advection_scheme="quick", cfl_target=0.5, cfl_growth=1.05, Re=1000.0,
                 lid_velocity=1.0, lin_tol=1e-6)
     st = O.allocate_state(n, n)
@@ -202,16 +221,126 @@ def cpu_baseline(n, iters_per_step, args):
         limiter.unregister()
     b = (times[40] - times[8]) / 32.0
     a = max(times[8] - 8 * b, 0.0)
+    return a, b
+
+
+def cpu_baseline(n, iters_per_step, args, iters_source):
+    """Time the FP64 NumPy oracle on this host (one BLAS thread, and again
+    with every core) and extrapolate to ``iters_per_step`` PCG iterations."""
+    a, b = _oracle_step_costs(n, 1)
     t_step = a + b * iters_per_step
+    a_all, b_all = _oracle_step_costs(n, None)
+    t_all = a_all + b_all * iters_per_step
+    ncpu = os.cpu_count()
     return {
         "value": round(n * n / t_step / 1e6, 6),
         "unit": "Mcell-steps/s",
         "cores": 1,
         "kind": "port",
+        "host_cpu_count": ncpu,
+        "all_cores": {"value": round(n * n / t_all / 1e6, 6), "cores": ncpu,
+                      "note": "same oracle with BLAS free to use every core; only its dot products and norms "
+                              "are threaded, the stencils are single-threaded NumPy"},
         "sample": (f"FP64 NumPy oracle, one {n}x{n} QUICK step (step 1 from rest) timed with the PCG capped at 8 and "
                    f"40 iterations: {a:.3f} s outside PCG + {b * 1e3:.2f} ms per PCG iteration, extrapolated to "
-                   f"the GPU run's mean {iters_per_step:.0f} iterations/step (BLAS limited to 1 thread)"),
+                   f"{iters_per_step:.0f} iterations/step ({iters_source}); BLAS limited to 1 thread"),
     }
+
+
+def c1_baselines(args):
+    """The 64^2 cavity run (Re 100, lid 1, 50 steps, lin_tol 1e-8, lin_maxiter
+    300, QUICK, FP64): the drop-in SimulationDriver.run on the GPU, and the FP64
+    oracle on this host in its vectorised form (i) and in the reference's
+    loop-faithful form (ii: per-cell Python advection and a per-step LIL
+    Jacobi assembly)."""
+    import numpy as np
+    from types import SimpleNamespace
+    import torch
+    from oracle import pyflow_oracle as O
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.drivers.simulation_driver import SimulationDriver
+    from pyflow.residuals.manager import ResidualManager
+    n, steps = 64, 50
+    kw = dict(Re=100.0, lid_velocity=1.0, cfl_target=0.5, cfl_growth=1.05, advection_scheme="quick",
+              disable_advection=False, lin_tol=1e-8, lin_maxiter=300)
+    cfg = SimpleNamespace(nx=n, ny=n, diagnostics=False, force_quiet=True, precision="fp64", **kw)
+    for _ in SimulationDriver(cfg, allocate_state(n, n), ResidualManager()).run(max_steps=2):
+        pass  # warm: first device allocation and code load
+    state = allocate_state(n, n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in SimulationDriver(cfg, state, ResidualManager()).run(max_steps=steps):
+        pass
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    u_gpu = state.fields["u"].copy()
+    ocfg = SimpleNamespace(**kw)
+    out = {}
+    for name, loops in (("vectorised", False), ("loop_faithful", True)):
+        st = O.allocate_state(n, n)
+        t0 = time.perf_counter()
+        for it in range(steps):
+            O.step(ocfg, st, it, loops=loops)
+        out[name] = time.perf_counter() - t0
+    err = float(np.max(np.abs(u_gpu - st.fields["u"])))
+    cells = n * n * steps
+    return {
+        "workload": "64x64 lid-driven cavity, Re 100, lid 1, 50 steps, lin_tol 1e-8, lin_maxiter 300, QUICK, FP64",
+        "gpu_drop_in": {"seconds": round(t_gpu, 4), "ms_per_step": round(t_gpu / steps * 1e3, 3),
+                        "mcell_steps_s": round(cells / t_gpu / 1e6, 4),
+                        "path": "SimulationDriver.run -> step -> libpyflow_hip (one device step per yield)"},
+        "cpu_vectorised": {"seconds": round(out["vectorised"], 3), "mcell_steps_s": round(cells / out["vectorised"] / 1e6, 4),
+                           "cores": 1, "kind": "port"},
+        "cpu_loop_faithful": {"seconds": round(out["loop_faithful"], 3),
+                              "mcell_steps_s": round(cells / out["loop_faithful"] / 1e6, 4), "cores": 1,
+                              "kind": "port"},
+        "max_abs_du_gpu_vs_oracle": err,
+        "speedup_vs_vectorised": round(out["vectorised"] / t_gpu, 2),
+        "speedup_vs_loop_faithful": round(out["loop_faithful"] / t_gpu, 2),
+    }
+
+
+def drop_in_c2(args, steps, tile):
+    """The 1024^2 cavity from rest for ``steps`` steps twice: through the
+    C-ABI batch run (pf_batch_run of all steps) and through the drop-in
+    SimulationDriver.run (one device step per yield). The difference is the
+    host cost the drop-in path adds per step: parameter arming, the scalar
+    state download, the diagnostics dictionaries, the centerline."""
+    from types import SimpleNamespace
+    import torch
+    from pyflow import autotune
+    from pyflow._device import DeviceBatch
+    from pyflow.core.ghost_fields import allocate_state
+    from pyflow.drivers.simulation_driver import SimulationDriver
+    from pyflow.residuals.manager import ResidualManager
+    n = args.n
+    params = c2_params(n, args.inner_rtol)
+    if tile is not None:
+        autotune.apply(params, tile)
+    batch = DeviceBatch([params], args.precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    batch.close()
+    cfg = SimpleNamespace(nx=n, ny=n, Re=1000.0, lid_velocity=1.0, cfl_target=0.5, cfl_growth=1.05,
+                          advection_scheme="quick", disable_advection=False, lin_tol=1e-6, lin_maxiter=10000,
+                          diagnostics=False, force_quiet=True, precision=args.precision,
+                          inner_rtol=args.inner_rtol)
+    if tile is not None:  # the drop-in cavity runs with the batch's tiling
+        cfg.tile_rows, cfg.max_workgroups, cfg.pcg_per_round = tile.tile_rows, tile.max_workgroups, tile.pcg_per_round
+    state = allocate_state(n, n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in SimulationDriver(cfg, state, ResidualManager()).run(max_steps=steps):
+        pass
+    torch.cuda.synchronize()
+    t_drv = time.perf_counter() - t0
+    return {"steps_from_rest": steps, "driver_ms_per_step": round(t_drv / steps * 1e3, 3),
+            "batch_run_ms_per_step": round(t_batch / steps * 1e3, 3),
+            "host_overhead_ms_per_step": round((t_drv - t_batch) / steps * 1e3, 3),
+            "driver_mcell_steps_s": round(n * n * steps / t_drv / 1e6, 3)}
 
 
 def main():
@@ -439,6 +568,7 @@ def run_cavity(args):
     pcg_time_s = (gpu_ms / 1e3) * min(1.0, rounds_per_iter * its / max(rounds, 1))
     pcg_gbs = pcg_bytes / max(pcg_time_s, 1e-30) / 1e9
     value = N * args.steps / wall / 1e6
+    traffic = measured_traffic(n, args.precision, tile)
 
     accuracy = None
     if not args.no_accuracy and args.precision == "mixed":
@@ -455,25 +585,41 @@ def run_cavity(args):
             "max_abs_du_over_U": float(np.max(np.abs(u32 - u64))),
             "max_abs_dv_over_U": float(np.max(np.abs(v32 - v64))),
             "rel_l2_dp": float(np.linalg.norm(p32 - p64) / max(np.linalg.norm(p64), 1e-300)),
+            "rel_l2_dgradp": _grad_rel(p32, p64),
             "continuity": float(d.continuity),
             "continuity_fp64": float(d64.continuity),
             "continuity_diff": float(abs(d.continuity - d64.continuity)),
             "pcg_iterations_fp64": int(d64.rp_iterations_run),
         }
         log(args, f"[bench] accuracy {accuracy}")
-        if args.oracle_steps > 0:
-            accuracy["vs_oracle"] = oracle_accuracy(n, params, args)
-            log(args, f"[bench] accuracy vs the FP64 NumPy oracle {accuracy['vs_oracle']}")
+    if not args.no_accuracy and args.oracle_steps > 0:
+        accuracy = accuracy or {}
+        accuracy["vs_oracle"] = oracle_accuracy(n, params, args)
+        log(args, f"[bench] accuracy vs the FP64 NumPy oracle {accuracy['vs_oracle']}")
 
+    # The baselines run the FP64 recurrence (the reference's), so they are
+    # extrapolated to the FP64 path's own iteration count when it was measured.
+    if accuracy and accuracy.get("pcg_iterations_fp64"):
+        base_iters = accuracy["pcg_iterations_fp64"] / args.steps
+        iters_source = "the FP64 fused path's mean over the same steps"
+    else:
+        base_iters = its / args.steps
+        iters_source = "the timed run's mean"
     cpu = None
     if not args.no_cpu_baseline:
         log(args, "[bench] CPU baseline (FP64 NumPy oracle, bounded sample)")
-        cpu = cpu_baseline(n, its / args.steps, args)
+        cpu = cpu_baseline(n, base_iters, args, iters_source)
+        cpu["speedup"] = round(value / cpu["value"], 1)
     plain = None
     if not args.no_plain:
         log(args, "[bench] plain FP64 GPU form (one PyTorch kernel per NumPy op), bounded sample")
-        plain = plain_fp64_baseline(n, its / args.steps)
+        plain = plain_fp64_baseline(n, base_iters)
         plain["speedup"] = round(value / plain["value"], 2)
+    drop_in = None
+    if not args.no_drop_in:
+        log(args, "[bench] drop-in path: SimulationDriver.run at 1024^2 and the 64^2 run with its CPU baselines")
+        drop_in = {"c2": drop_in_c2(args, 20, tile), "c1": c1_baselines(args)}
+        log(args, f"[bench] drop-in {drop_in}")
 
     out = {
         "metric": METRIC,
@@ -499,16 +645,19 @@ def run_cavity(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n, args.precision, tile),
+            "traffic": traffic,
+            "actual_frac": None if traffic is None else round(traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
             "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "alg_bytes_per_launch": int(alg_bytes / max(rounds, 1)),
             "launches": rounds,
+            "launches_per_step": round(rounds / args.steps, 1),
         },
         "pcg": {"iterations": its, "refinement_passes": outer, "gbs": round(pcg_gbs, 2),
                 "hbm_frac": round(pcg_gbs / HBM_PEAK_GBS, 4), "iters_per_step": round(its / args.steps, 1)},
         "accuracy": accuracy,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
+        "drop_in": drop_in,
     }
     print(json.dumps(out), flush=True)
     batch.close()
@@ -627,19 +776,6 @@ def run_poisson(args):
     return 0
 
 
-def batch_params(count, seed, inner_rtol):
-    """256 cavities (seed 0): n_i = 32 * U{1..16}, Re_i log-uniform in [100, 5000]."""
-    import numpy as np
-    from pyflow._device import cavity_params
-    rng = np.random.default_rng(seed)
-    ns = 32 * rng.integers(1, 17, size=count)
-    res = np.exp(rng.uniform(np.log(100.0), np.log(5000.0), size=count))
-    params = [cavity_params(int(n), int(n), dx=1.0, dy=1.0, scheme="quick", lid_velocity=1.0, Re=float(re),
-                            cfl_target=0.5, cfl_growth=1.05, lin_tol=1e-6, lin_maxiter=10000, jacobi=True,
-                            inner_rtol=inner_rtol) for n, re in zip(ns, res)]
-    return params, ns
-
-
 # The 256-cavity batch: the autotuner picks the region height and the
 # per-cavity workgroup cap for the whole batch (two-tile regions, one per
 # workgroup, run two workgroups per CU; several regions per workgroup make a
@@ -667,6 +803,7 @@ def run_batch(args):
     from pyflow import autotune
     from pyflow._device import DeviceBatch
     steps = args.steps
+    from pyflow.workloads import batch_params
     params, ns = batch_params(256, 0, args.inner_rtol)
     cells = int(np.sum(ns.astype(np.int64) ** 2))
     rows, cap, tuned = 0, MANY_CAVITY_MAXWG, None
@@ -731,18 +868,7 @@ def run_batch(args):
     return 0
 
 
-def serve_requests(count, seed, step_scale):
-    """Requests (seed): n ~ round(exp(U(ln 64, ln 2048))), Re log-uniform in
-    [100, 1e4], steps ~ U{100..2000} times step_scale."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    ns = np.rint(np.exp(rng.uniform(np.log(64.0), np.log(2048.0), size=count))).astype(int)
-    res = np.exp(rng.uniform(np.log(100.0), np.log(1e4), size=count))
-    steps = np.maximum(1, np.rint(rng.integers(100, 2001, size=count) * step_scale)).astype(int)
-    return ns, res, steps
-
-
-def serve(reqs, arrivals, precision, inner_rtol, chunk_rounds=64, max_cav=64, log_fn=None):
+def serve(reqs, arrivals, precision, inner_rtol, hbm_budget, chunk_rounds=64, max_cav=64, log_fn=None):
     """Continuous batching through pyflow.serving.CavityServer: requests are
     submitted at their arrival times, admitted by HBM footprint into the
     running batch, advanced in chunks and retired when done. Returns
@@ -751,7 +877,8 @@ def serve(reqs, arrivals, precision, inner_rtol, chunk_rounds=64, max_cav=64, lo
     import collections
     from pyflow.serving import CavityRequest, CavityServer
     ns, res, steps = reqs
-    srv = CavityServer(precision, max_cavities=max_cav, chunk_rounds=chunk_rounds, inner_rtol=inner_rtol)
+    srv = CavityServer(precision, hbm_budget=hbm_budget, max_cavities=max_cav, chunk_rounds=chunk_rounds,
+                       inner_rtol=inner_rtol)
     order = collections.deque(sorted(range(len(ns)), key=lambda k: arrivals[k]))
     ticket_of, done = {}, {}
     t0 = time.perf_counter()
@@ -772,15 +899,17 @@ def serve(reqs, arrivals, precision, inner_rtol, chunk_rounds=64, max_cav=64, lo
             last_log = time.perf_counter()
             log_fn(f"[bench] serving: {len(done)} done, {srv.running} running, {srv.waiting + len(order)} waiting")
     makespan = time.perf_counter() - t0
-    peak = (srv.peak_running, srv.peak_bytes)
+    peak = (srv.peak_running, srv.peak_bytes, srv.evicted)
     srv.close()
     return done, makespan, peak
 
 
 def run_serve(args):
     import numpy as np
-    import torch  # noqa: F401
+    import torch
+    from pyflow.workloads import poisson_arrivals, serve_requests
     count = args.requests
+    budget = int(args.hbm_budget_gib * (1 << 30))
     reqs = serve_requests(count, 0, args.step_scale)
     ns, res, steps = reqs
     work = ns.astype(np.float64) ** 2 * steps
@@ -788,13 +917,15 @@ def run_serve(args):
     m = min(8, count)
     log(args, f"[bench] capacity probe: {m} requests at once")
     cal = (ns[:m], res[:m], steps[:m])
-    _, span, _ = serve(cal, np.zeros(m), args.precision, args.inner_rtol)
+    _, span, _ = serve(cal, np.zeros(m), args.precision, args.inner_rtol, budget, log_fn=lambda m_: log(args, m_))
     capacity = float(np.sum(work[:m])) / span  # cell-steps/s
     lam = 0.7 * capacity / float(np.mean(work))  # requests/s at 70% of capacity
-    arrivals = np.cumsum(np.random.default_rng(1).exponential(1.0 / lam, size=count))
+    arrivals = poisson_arrivals(count, lam, seed=1)
     log(args, f"[bench] capacity {capacity / 1e6:.1f} Mcell-steps/s; arrival rate {lam:.3f}/s; {count} requests")
-    done, makespan, (peak_running, peak_bytes) = serve(reqs, arrivals, args.precision, args.inner_rtol,
-                                                       log_fn=lambda m_: log(args, m_))
+    torch.cuda.reset_peak_memory_stats()
+    done, makespan, (peak_running, peak_bytes, evicted) = serve(reqs, arrivals, args.precision, args.inner_rtol,
+                                                                budget, log_fn=lambda m_: log(args, m_))
+    reserved = torch.cuda.max_memory_reserved()
     lat = np.array([done[k] - arrivals[k] for k in range(count)])
     out = {
         "metric": "p50/p99 served-solve latency (continuous batching of lid-driven cavity requests)",
@@ -806,8 +937,11 @@ def run_serve(args):
                  f"ln 2048))), Re log-uniform [100, 1e4], steps ~ U{{100..2000}} x {args.step_scale} (seed 0)"),
         "config": {"workload": f"{count} cavity requests, QUICK, tol 1e-6, lin_maxiter 10000", "requests": count,
                    "arrival_rate_per_s": round(lam, 4), "precision": args.precision,
-                   "admission": "HBM footprint, first fit in arrival order (pyflow.serving.CavityServer)",
-                   "peak_running": peak_running, "peak_device_bytes": int(peak_bytes)},
+                   "admission": "HBM footprint, first fit in arrival order, in one arena of the budget "
+                                "(pyflow.serving.CavityServer)",
+                   "hbm_budget_bytes": budget, "step_scale": args.step_scale,
+                   "peak_running": peak_running, "peak_workspace_bytes": int(peak_bytes),
+                   "torch_max_reserved_bytes": int(reserved), "evicted_nan": evicted},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "mean": round(float(np.mean(lat)), 4), "max": round(float(np.max(lat)), 4)},
         "throughput_mcell_steps_s": round(float(np.sum(work)) / makespan / 1e6, 3),

@@ -39,7 +39,8 @@ import numpy as np
 
 __all__ = [
     "gradient_coherent", "divergence_coherent", "laplacian_coherent",
-    "divergence_central", "advect_quick", "advect_upwind",
+    "divergence_central", "advect_quick", "advect_upwind", "advect_quick_loops",
+    "advect_upwind_loops", "jacobi_inv_diag_assembled",
     "neg_laplacian_pinned", "pos_laplacian_dirichlet", "jacobi_inv_diag",
     "assemble_negative_laplacian_diag", "cg", "SolveResult", "solve",
     "solve_pressure_poisson", "OracleState", "allocate_state", "step",
@@ -167,6 +168,78 @@ def advect_upwind(u, v, f, dx, dy):
 
 
 # --------------------------------------------------------------------------
+# Loop-faithful forms: the reference's cost shape, one Python iteration per
+# cell (advection.py:17-84 loops over j, i; pressure_solver.py:15-40 assembles
+# a LIL matrix cell by cell on every step). Same per-cell arithmetic as the
+# vectorised forms above, so the results are bitwise equal (tested); they
+# exist to time the reference's own algorithm on the host (CPU baseline ii).
+# --------------------------------------------------------------------------
+def advect_quick_loops(u, v, f, dx, dy):
+    ny, nx = f.shape
+    ph = np.pad(f, 2, mode="edge")
+    out = np.zeros_like(f)
+    for j in range(ny):
+        for i in range(nx):
+            r, c = j + 2, i + 2
+            uc, vc = u[j, i], v[j, i]
+            if uc >= 0:
+                fe = (_C3 * ph[r, c + 1] + _C6 * ph[r, c]) - _C1 * ph[r, c - 1]
+                fw = (_C3 * ph[r, c] + _C6 * ph[r, c - 1]) - _C1 * ph[r, c - 2]
+            else:
+                fe = (_C3 * ph[r, c - 1] + _C6 * ph[r, c]) - _C1 * ph[r, c + 1]
+                fw = (_C3 * ph[r, c - 2] + _C6 * ph[r, c - 1]) - _C1 * ph[r, c]
+            if vc >= 0:
+                fn = (_C3 * ph[r + 1, c] + _C6 * ph[r, c]) - _C1 * ph[r - 1, c]
+                fs = (_C3 * ph[r, c] + _C6 * ph[r - 1, c]) - _C1 * ph[r - 2, c]
+            else:
+                fn = (_C3 * ph[r - 1, c] + _C6 * ph[r, c]) - _C1 * ph[r + 1, c]
+                fs = (_C3 * ph[r - 2, c] + _C6 * ph[r - 1, c]) - _C1 * ph[r, c]
+            out[j, i] = (uc * fe - uc * fw) / dx + (vc * fn - vc * fs) / dy
+    return out
+
+
+def advect_upwind_loops(u, v, f, dx, dy):
+    ny, nx = f.shape
+    out = np.zeros_like(f)
+    for j in range(ny):
+        for i in range(nx):
+            c = f[j, i]
+            if u[j, i] > 0:
+                fx = (c - f[j, i - 1]) / dx if i > 0 else (c - c) / dx
+            else:
+                fx = (f[j, min(i + 1, nx - 1)] - c) / dx
+            if v[j, i] > 0:
+                fy = (c - f[j - 1, i]) / dy if j > 0 else (c - c) / dy
+            else:
+                fy = (f[min(j + 1, ny - 1), i] - c) / dy
+            out[j, i] = u[j, i] * fx + v[j, i] * fy
+    return out
+
+
+def jacobi_inv_diag_assembled(nx: int, ny: int, dx: float, dy: float):
+    """Jacobi weights the way the reference obtains them every step: a LIL
+    matrix filled cell by cell (identity rows on the boundary, 5-point rows
+    inside), converted to CSR, its diagonal inverted."""
+    import scipy.sparse as sp
+    A = sp.lil_matrix((nx * ny, nx * ny))
+    cx, cy = 1.0 / dx ** 2, 1.0 / dy ** 2
+    for j in range(ny):
+        for i in range(nx):
+            k = j * nx + i
+            if i in (0, nx - 1) or j in (0, ny - 1):
+                A[k, k] = 1.0
+                continue
+            A[k, k] = 2.0 / dx ** 2 + 2.0 / dy ** 2
+            A[k, k - 1] = -cx
+            A[k, k + 1] = -cx
+            A[k, k - nx] = -cy
+            A[k, k + nx] = -cy
+    D = np.asarray(A.tocsr().diagonal()).ravel()
+    with np.errstate(divide="ignore"):
+        return np.where(D != 0, 1.0 / D, 0.0)
+
+
+# --------------------------------------------------------------------------
 # Pressure operators and Jacobi (reference: solvers/pressure_solver.py:15-106,
 # linear_solvers/preconditioners.py:8-25)
 # --------------------------------------------------------------------------
@@ -284,7 +357,7 @@ def solve(matvec, b, tol, maxiter, psolve=None):
 # Pressure projection (reference: solvers/pressure_solver.py:42-219)
 # --------------------------------------------------------------------------
 def solve_pressure_poisson(u, v, p, dt, dx, dy, cfg, rhs_override=None,
-                           use_jacobi=None):
+                           use_jacobi=None, loops=False):
     """In-place projection on interior arrays ``u, v, p`` of shape (ny, nx).
 
     Returns ``(p_solution, diagnostics)`` with the reference's keys plus
@@ -317,7 +390,7 @@ def solve_pressure_poisson(u, v, p, dt, dx, dy, cfg, rhs_override=None,
             use_jacobi = getattr(cfg, "enable_jacobi_pc", True)
         psolve = None
         if use_jacobi:
-            invD = jacobi_inv_diag(nx, ny, dx, dy)
+            invD = (jacobi_inv_diag_assembled if loops else jacobi_inv_diag)(nx, ny, dx, dy)
             psolve = lambda r: invD * r  # noqa: E731
         res = solve(mv, rhs, lin_tol, lin_maxiter, psolve)
         psol = res.x.reshape(ny, nx)
@@ -376,14 +449,16 @@ def _interior(a):
     return a[1:-1, 1:-1]
 
 
-def step(cfg: Any, state: OracleState, iteration: int = 0, ru_first: float = 1.0):
+def step(cfg: Any, state: OracleState, iteration: int = 0, ru_first: float = 1.0, loops: bool = False):
     """One projection step; returns ``(residuals, diagnostics)``.
 
     Follows ``solver.py:15-156``: dt controller (CFL limit, diffusion limit,
     growth cap, divergence-rise safeguard), predictor ``u* = u - dt*conv``,
     projection, lid BC after projection, ``Ru = ||du||/N``, continuity
     ``||central_div(u,v;1,1)||/N``, NaN flag. ``ru_first`` is the tracker's
-    first ``Ru`` value (the reference seeds it with 1.0).
+    first ``Ru`` value (the reference seeds it with 1.0). ``loops`` selects the
+    loop-faithful advection and Jacobi assembly (same results, the
+    reference's cost).
     """
     u = _interior(state.fields["u"])
     v = _interior(state.fields["v"])
@@ -426,12 +501,15 @@ def step(cfg: Any, state: OracleState, iteration: int = 0, ru_first: float = 1.0
         us, vs = u_old.copy(), v_old.copy()
     else:
         scheme = getattr(cfg, "advection_scheme", "upwind")
-        adv = advect_quick if scheme == "quick" else advect_upwind
+        if loops:
+            adv = advect_quick_loops if scheme == "quick" else advect_upwind_loops
+        else:
+            adv = advect_quick if scheme == "quick" else advect_upwind
         us = u_old - dt * adv(u_old, v_old, u_old, dx, dy)
         vs = v_old - dt * adv(u_old, v_old, v_old, dx, dy)
     u[:, :] = us
     v[:, :] = vs
-    _, pdiag = solve_pressure_poisson(u, v, p, dt, dx, dy, cfg)
+    _, pdiag = solve_pressure_poisson(u, v, p, dt, dx, dy, cfg, loops=loops)
     lid = getattr(cfg, "lid_velocity", None)
     if lid is not None:
         state.fields["u"][-2, 1:-1] = lid

@@ -96,11 +96,15 @@ struct pf_batch {
   hipEvent_t ev = nullptr;
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;  // timing of the pending chunk
   bool chunk_pending = false;
+  bool scal_fresh = false;  // hscal equals the device's latest state (pulled, nothing launched since)
   bool serving = false;  // chunks queued by pf_batch_advance: poll queues nothing more
   int64_t poll_chunk = 16;  // rounds queued by the next pf_batch_poll (doubles up to 256)
   bool timing = false;
   double gpu_ms = 0.0;       // summed GPU time of launched chunks (timing on)
   int64_t timed_rounds = 0;
+  double* dgather = nullptr;  // device column buffer (pf_batch_centerline), grown on demand
+  double* hgather = nullptr;  // pinned host copy of it
+  int gather_cap = 0;
 };
 
 static size_t esize(int prec) { return prec == PF_PREC_MIXED ? sizeof(float) : sizeof(double); }
@@ -298,6 +302,7 @@ static int pull_scal(pf_batch* b) {
   HIPCHK(hipMemcpyAsync(b->hscal, b->dscal + (size_t)par * b->max_cav, sizeof(PfScal) * b->max_cav,
                         hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  b->scal_fresh = true;
   return PF_OK;
 }
 
@@ -716,6 +721,8 @@ int pf_batch_destroy(pf_batch* b) {
   if (b->ev) (void)hipEventDestroy(b->ev);
   if (b->ev_t0) (void)hipEventDestroy(b->ev_t0);
   if (b->ev_t1) (void)hipEventDestroy(b->ev_t1);
+  if (b->dgather) (void)hipFree(b->dgather);
+  if (b->hgather) (void)hipHostFree(b->hgather);
   delete b;
   return PF_OK;
 }
@@ -729,15 +736,21 @@ int pf_batch_set_params(pf_batch* b, int slot, const pf_cavity_params* params) {
   if (params->nx != c.nx || params->ny != c.ny) return fail(PF_EINVAL, "nx/ny cannot change");
   // The workspace (partials: 2 x nblk slots) was carved for the blocking
   // chosen at creation; keep it whatever the environment says now.
-  const int nblk = c.nblk, tpb = c.tpb, nreg = c.nreg, rpw = c.rpw;
-  fill_params(c, params, b->prec);
-  c.nblk = nblk;
-  c.tpb = tpb;
-  c.nreg = nreg;
-  c.rpw = rpw;
-  choose_ca(c, b->prec, params->pcg_per_round);
+  PfCav nc;
+  std::memcpy(&nc, &c, sizeof(PfCav));
+  fill_params(nc, params, b->prec);
+  nc.nblk = c.nblk;
+  nc.tpb = c.tpb;
+  nc.nreg = c.nreg;
+  nc.rpw = c.rpw;
+  choose_ca(nc, b->prec, params->pcg_per_round);
   b->hpar[slot] = *params;
-  b->tables_dirty = true;
+  // Unchanged parameters (the drop-in step loop sets them before every step)
+  // leave the device tables and the instantiated graphs as they are.
+  if (std::memcmp(&nc, &c, sizeof(PfCav)) != 0) {
+    std::memcpy(&c, &nc, sizeof(PfCav));
+    b->tables_dirty = true;
+  }
   return PF_OK;
 }
 
@@ -811,11 +824,16 @@ int pf_batch_set_meta(pf_batch* b, int slot, double dt_prev, const double* hist,
   if (rc) return rc;
   if (hist_len < 0 || (hist_len > 0 && !hist)) return fail(PF_EINVAL, "bad history");
   PfScal& s = b->hscal[slot];
-  s.dt_prev = dt_prev;
   // keep the last 5 entries, as hist[-5:] does
   const int n = hist_len > 5 ? 5 : hist_len;
-  for (int k = 0; k < 5; ++k) s.hist[k] = 0.0;
-  for (int k = 0; k < n; ++k) s.hist[k] = hist[hist_len - n + k];
+  double h[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < n; ++k) h[k] = hist[hist_len - n + k];
+  // The drop-in step loop passes back what the last step left on the device:
+  // then there is nothing to upload.
+  if (b->scal_fresh && s.dt_prev == dt_prev && s.hist_len == n && std::memcmp(s.hist, h, sizeof(h)) == 0)
+    return PF_OK;
+  s.dt_prev = dt_prev;
+  std::memcpy(s.hist, h, sizeof(h));
   s.hist_len = n;
   return push_scal(b, slot);
 }
@@ -872,6 +890,7 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
     sc.steps_done = 0;
     sc.it_run = 0;
     sc.outer_run = 0;
+    sc.rounds_run = 0;
     if (c.mode == PF_MODE_STEP) {
       sc.phase = PH_STATS;
       sc.steps_left = nsteps;
@@ -897,6 +916,7 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
 }
 
 static int chunk(pf_batch* b, int64_t n) {
+  b->scal_fresh = false;
   if (b->timing) HIPCHK(hipEventRecord(b->ev_t0, b->stream));
   const int rc = launch_rounds(b, n);
   if (rc) return rc;
@@ -934,10 +954,17 @@ int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* d
       if (rc) return rc;
       step = std::min<int64_t>(step * 2, 256);
     }
-    b->predicted_rounds = b->rounds;
   }
   rc = pull_scal(b);
   if (rc) return rc;
+  if (*b->hactive <= 0) {
+    // The next run's first chunk: the rounds the slowest cavity needed this
+    // time (the launched rounds include the last chunk's overshoot).
+    int64_t need = 0;
+    for (int s = 0; s < b->max_cav; ++s)
+      if (b->used[s]) need = std::max<int64_t>(need, b->hscal[s].rounds_run);
+    if (need > 0) b->predicted_rounds = need;
+  }
   if (diag)
     for (int s = 0; s < b->max_cav; ++s)
       if (b->used[s]) fill_diag(b, s, &diag[s]);
@@ -1007,6 +1034,7 @@ int pf_batch_start(pf_batch* b, int slot, int nsteps) {
   sc.steps_done = 0;
   sc.it_run = 0;
   sc.outer_run = 0;
+  sc.rounds_run = 0;
   if (c.mode == PF_MODE_STEP) {
     sc.phase = PH_STATS;
     sc.steps_left = nsteps;
@@ -1055,14 +1083,28 @@ int pf_batch_centerline(pf_batch* b, int slot, double* out) {
   if (rc) return rc;
   PfCav& c = b->hcav[slot];
   if (!out) return fail(PF_EINVAL, "out is NULL");
+  if (c.ny > b->gather_cap) {
+    if (b->dgather) HIPCHK(hipFree(b->dgather));
+    if (b->hgather) HIPCHK(hipHostFree(b->hgather));
+    b->dgather = nullptr;
+    b->hgather = nullptr;
+    b->gather_cap = 0;
+    HIPCHK(hipMalloc((void**)&b->dgather, sizeof(double) * c.ny));
+    HIPCHK(hipHostMalloc((void**)&b->hgather, sizeof(double) * c.ny, hipHostMallocDefault));
+    b->gather_cap = c.ny;
+  }
+  // One gather of u[:, nx//2] into a contiguous buffer and one copy: a
+  // strided copy of ny single elements costs milliseconds at 1024 rows.
   const int cx = c.nx / 2;
-  const size_t es = esize(b->prec);
-  std::vector<char> tmp((size_t)c.ny * es);
-  HIPCHK(hipMemcpy2DAsync(tmp.data(), es, (const char*)c.u + cx * es, es * c.nx, es, c.ny, hipMemcpyDeviceToHost,
-                          b->stream));
+  const dim3 grid((unsigned)((c.ny + 255) / 256)), blk(256);
+  if (b->prec == PF_PREC_MIXED)
+    k_column<float><<<grid, blk, 0, b->stream>>>((const float*)c.u, c.nx, c.ny, cx, b->dgather);
+  else
+    k_column<double><<<grid, blk, 0, b->stream>>>((const double*)c.u, c.nx, c.ny, cx, b->dgather);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->hgather, b->dgather, sizeof(double) * c.ny, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
-  for (int j = 0; j < c.ny; ++j)
-    out[j] = (es == sizeof(float)) ? (double)((const float*)tmp.data())[j] : ((const double*)tmp.data())[j];
+  std::memcpy(out, b->hgather, sizeof(double) * c.ny);
   return PF_OK;
 }
 

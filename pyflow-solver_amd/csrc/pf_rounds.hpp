@@ -97,7 +97,8 @@ struct PfScal {
   int pending;   // phase whose partials await their tail (0 = none)
   int steps_left, steps_done, it, it_total, info, outer, x_is_b;
   int hist_len, safeguard, nan, converged, it_run, outer_run, done_now;
-  int ca_inc, ca_pad;  // two-iteration round: iterations the pending round runs (0: basis only)
+  int ca_inc;      // two-iteration round: iterations the pending round runs (0: basis only)
+  int rounds_run;  // rounds this cavity executed a phase in since it was armed
   double dt_prev, dt, cfl_now, umax, vmax, hist[5], div_before;
   double mean, div_initial, bnorm, atol, inner_atol, rho, rho_prev, beta, pq, alpha, rnorm2, rn_outer;
   double rnorm_true, div_after, ru, rv, continuity;
@@ -1654,7 +1655,10 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   }
   if (b == 0) {
     __syncthreads();
-    if (tid == 0) st.pending = phase;
+    if (tid == 0) {
+      st.pending = phase;
+      st.rounds_run += 1;
+    }
     __syncthreads();
     if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
   }
@@ -1763,7 +1767,10 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   }
   if (b == 0) {
     __syncthreads();
-    if (tid == 0) st.pending = phase;
+    if (tid == 0) {
+      st.pending = phase;
+      st.rounds_run += 1;
+    }
     __syncthreads();
     if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
   }
@@ -1898,6 +1905,13 @@ __global__ void k_op(int opk, int nx, int ny, double dx, double dy, const double
     case PF_OPK_POS_LAP_DIRICHLET: out0[k] = apply_op_at<double>(1, A, nx, ny, j, i, dx, dy); break;
     default: break;
   }
+}
+
+// One column of a cavity field as float64 (the driver's centerline).
+template <class T>
+__global__ void k_column(const T* __restrict__ src, int nx, int ny, int col, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < ny) out[j] = (double)src[(size_t)j * nx + col];
 }
 
 template <class T>

@@ -32,8 +32,14 @@ def _grid(state: State, config):
 
 
 def step_params(config, nx: int, ny: int, dx: float, dy: float, mode: int = H.MODE_STEP) -> H.CavityParams:
-    """Device parameters from the attributes ``step`` reads (solver.py:16-97, pressure_solver.py:134-139)."""
-    return cavity_params(
+    """Device parameters from the attributes ``step`` reads (solver.py:16-97, pressure_solver.py:134-139).
+
+    Optional attributes a config may add (the reference has none of them):
+    ``precision`` ("fp64" / "mixed"), ``inner_rtol``, and the tiling
+    ``tile_rows``, ``max_workgroups``, ``pcg_per_round`` (0 or absent: the
+    library's default). A state that already ran keeps the tiling it first
+    ran with (``state.meta['kernel_config']``)."""
+    p = cavity_params(
         nx, ny, dx=dx, dy=dy,
         scheme=getattr(config, "advection_scheme", "upwind"),
         disable_advection=bool(getattr(config, "disable_advection", False)),
@@ -47,6 +53,9 @@ def step_params(config, nx: int, ny: int, dx: float, dy: float, mode: int = H.MO
         mode=mode,
         inner_rtol=float(getattr(config, "inner_rtol", D.DEFAULT_INNER_RTOL)),
     )
+    for k in ("tile_rows", "max_workgroups", "pcg_per_round"):
+        setattr(p, k, int(getattr(config, k, 0) or 0))
+    return p
 
 
 def step(config, state: State, tracker, iteration: int) -> tuple[State, dict[str, float], dict[str, Any]]:

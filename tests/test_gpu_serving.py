@@ -62,31 +62,30 @@ def test_server_refuses_request_larger_than_budget():
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_nan_request_is_evicted_without_disturbing_neighbours(precision):
-    """One running request's u gets a NaN between two chunks: the server
-    evicts it once its flag shows (status nan_detected, fewer steps than
-    asked, workspace released), and the co-resident requests still give the
-    same bits as alone."""
+    """A request whose fields hold a NaN joins a running batch: the server
+    evicts it at the first chunk boundary after its first step started
+    (status nan_detected, no step completed, its arena block released) -- in
+    the reference's semantics its every PCG solve would run to lin_maxiter --
+    and the co-resident requests still give the same bits as alone."""
     from pyflow.serving import CavityRequest, CavityServer, footprint
-    reqs = [CavityRequest(96, 80, 6, Re=400.0), CavityRequest(64, 64, 40, Re=1000.0),
-            CavityRequest(130, 70, 5, Re=150.0)]
+    nb = CavityRequest(64, 64, 40, Re=1000.0)
+    u = np.zeros((nb.ny + 2, nb.nx + 2))
+    u[nb.ny // 2, nb.nx // 3] = np.nan
+    nb.fields = (u, np.zeros_like(u), np.zeros_like(u))
+    reqs = [CavityRequest(96, 80, 6, Re=400.0), nb, CavityRequest(130, 70, 5, Re=150.0)]
     budget = sum(footprint(r, precision) for r in reqs)
     srv = CavityServer(precision, hbm_budget=budget, max_cavities=4, chunk_rounds=16)
-    tickets = [srv.submit(r) for r in reqs]
+    tickets = [srv.submit(reqs[0])]
     results = {r.ticket: r for r in srv.pump()}
-    slot = next(s for s, e in srv._running.items() if e.ticket == tickets[1])
-    n = reqs[1]
-    u = np.zeros((n.ny + 2, n.nx + 2))
-    srv.batch.download(slot, u, None, None)
-    u[n.ny // 2, n.nx // 3] = np.nan
-    srv.batch.upload(slot, u, None, None)
-    print(f"[serve nan] {precision}: NaN injected into request {tickets[1]} (slot {slot})", flush=True)
+    tickets += [srv.submit(r) for r in reqs[1:]]  # the NaN request arrives while another runs
+    print(f"[serve nan] {precision}: request {tickets[1]} submitted with a NaN in u", flush=True)
     while not srv.idle():
         for r in srv.pump():
             results[r.ticket] = r
     assert srv.evicted == 1 and srv.in_use_bytes == 0 and srv.arena.free_bytes == srv.arena.size
     bad = results[tickets[1]]
-    assert bad.status == "nan_detected" and bad.diag.steps_done < n.steps
-    assert np.isnan(bad.u).any()
+    assert bad.status == "nan_detected" and bad.diag.steps_done == 0 and bad.diag.rp_iterations_run < 2 * 16
+    assert np.isnan(bad.diag.umax) and np.isnan(bad.u).any()
     for t, req in ((tickets[0], reqs[0]), (tickets[2], reqs[2])):
         r = results[t]
         assert r.status == "done"
