@@ -816,13 +816,27 @@ __device__ inline void cell_corr_a(const PfCav& C, const PhaseArgs& A, int j, in
   const int nx = C.nx, ny = C.ny;
   const size_t k = (size_t)j * nx + i;
   Grid<T> PG{(const T*)C.p, nx};
-  const T dt = (T)A.dt;
-  const T gx = grad_x_coherent<T>(PG, nx, ny, j, i, (T)C.dx);
-  const T gy = grad_y_coherent<T>(PG, nx, ny, j, i, (T)C.dy);
   T* US = (T*)C.us;
   T* VS = (T*)C.vs;
-  const T uc = US[k] - dt * gx;
-  const T vc = VS[k] - dt * gy;
+  T uc, vc;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    const T dt = (T)A.dt;
+    const T gx = grad_x_coherent<T>(PG, nx, ny, j, i, (T)C.dx);
+    const T gy = grad_y_coherent<T>(PG, nx, ny, j, i, (T)C.dy);
+    uc = US[k] - dt * gx;
+    vc = VS[k] - dt * gy;
+  } else {
+    // MIXED: the gradient of the FP64 solution p = x - x[0] (FIN stores p
+    // rounded to float32; near the lid corners of an impulsive start p is
+    // large and its float32 rounding would move u by dt * |p| * 2^-24).
+    const double* XS = A.x_is_b ? C.b : C.x64;
+    const double x0 = A.x0;
+    auto P64 = [&](int jj, int ii) -> double { return XS[(size_t)jj * nx + ii] - x0; };
+    const double gx = grad_x_coherent<double>(P64, nx, ny, j, i, C.dx);
+    const double gy = grad_y_coherent<double>(P64, nx, ny, j, i, C.dy);
+    uc = (T)((double)US[k] - A.dt * gx);
+    vc = (T)((double)VS[k] - A.dt * gy);
+  }
   US[k] = uc;
   VS[k] = vc;
   const T uf = (C.lid_on && j == ny - 1) ? (T)C.lid : uc;
@@ -1500,7 +1514,7 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
   PhaseArgs A = A0;
   // FIN: p = x - x[0] (pressure_solver.py:147-149); the solution is b itself
   // when the right-hand side was zero.
-  if (phase == PH_FIN) A.x0 = A.x_is_b ? C.b[0] : C.x64[0];
+  if (phase == PH_FIN || (phase == PH_CORR_A && C.prec == PF_PREC_MIXED)) A.x0 = A.x_is_b ? C.b[0] : C.x64[0];
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < C.rpw; ++k) {
     const int r = b * C.rpw + k;
