@@ -235,6 +235,24 @@ int pf_op_apply(int opk, int nx, int ny, double dx, double dy,
                 const double* in0, const double* in1, const double* in2,
                 double* out0, double* out1, void* stream);
 
+/* Allocation-free solve. No counterpart in the reference: it serves callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid hipMalloc). It is the PF_MODE_SOLVE run of pf_batch_create +
+ * pf_batch_set_rhs + pf_batch_run + pf_batch_get_solution on one cavity
+ * (linear_solvers.solve on a PyFlow operator, reference
+ * linear_solvers/interface.py:23-53): the same kernels and tiling, so the same
+ * bits. `rhs` (read only) and `x` (the FP64 solution, written) are device
+ * arrays of ny x nx float64 in dense rows; every other buffer lives in a
+ * static device array of PF_NOALLOC_SCRATCH_BYTES, one per GPU, held until
+ * the process ends. A cavity whose buffers do not fit returns PF_EINVAL. No
+ * hipMalloc, hipFree, event or graph is created. Blocks until the solve is
+ * done (it polls its stream, so it cannot be captured into a graph). Calls
+ * that overlap on one GPU race on that GPU's scratch array. `diag` may be
+ * NULL. */
+#define PF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double* rhs, double* x,
+                     void* stream, pf_step_diag* diag);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -445,8 +446,7 @@ static int launch_rounds(pf_batch* b, int64_t n) {
   return PF_OK;
 }
 
-static void fill_diag(const pf_batch* b, int slot, pf_step_diag* d) {
-  const PfScal& s = b->hscal[slot];
+static void diag_from(const PfScal& s, pf_step_diag* d) {
   std::memset(d, 0, sizeof(*d));
   d->dt = s.dt;
   d->cfl = s.cfl_now;
@@ -474,6 +474,8 @@ static void fill_diag(const pf_batch* b, int slot, pf_step_diag* d) {
   d->rp_iterations_run = s.it_run;
   d->outer_passes_run = s.outer_run;
 }
+
+static void fill_diag(const pf_batch* b, int slot, pf_step_diag* d) { diag_from(b->hscal[slot], d); }
 
 // Host <-> device field copies between runs. FP64 fields are copied in
 // place; MIXED fields convert through an FP64 staging area, the cavity's
@@ -516,6 +518,34 @@ static int download_field(pf_batch* b, PfCav& c, const T* src, double* dst, int6
   return PF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free solve (pf_solve_noalloc): every buffer of the solve except
+// the caller's right-hand side and solution lives in this array. Each GPU the
+// library runs on holds its own copy for the life of the process; its device
+// address is looked up once per device (hipGetSymbolAddress) and kept.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[PF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int PF_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[PF_MAX_DEVICES] = {};
+
+// The current device's address of `scratch` (resolved on its first use).
+static int scratch_for_current_device(char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= PF_MAX_DEVICES) return fail(PF_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(PF_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = (char*)g_scratch_ptr[dev];
+  return PF_OK;
+}
+
 extern "C" {
 
 int pf_abi_version(void) { return PF_ABI_VERSION; }
@@ -527,7 +557,8 @@ size_t pf_sizeof_diag(void) { return sizeof(pf_step_diag); }
 
 int pf_device_init(int device) {
   HIPCHK(hipSetDevice(device));
-  return PF_OK;
+  char* scr = nullptr;  // resolve this device's allocation-free scratch address now
+  return scratch_for_current_device(&scr);
 }
 
 int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* params, size_t* bytes) {
@@ -1150,6 +1181,121 @@ int pf_op_apply(int opk, int nx, int ny, double dx, double dy, const double* in0
   dim3 grid((nx + TX - 1) / TX, (ny + TY - 1) / TY);
   k_op<<<grid, dim3(TX, TY), 0, (hipStream_t)stream>>>(opk, nx, ny, dx, dy, in0, in1, in2, out0, out1);
   HIPCHK(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double* rhs, double* x, void* stream,
+                     pf_step_diag* diag) {
+  if (!rhs || !x) return fail(PF_EINVAL, "rhs and x must be device pointers");
+  if (precision != PF_PREC_FP64 && precision != PF_PREC_MIXED) return fail(PF_EINVAL, "bad precision");
+  int rc = check_params(params);
+  if (rc) return rc;
+  if (params->mode != PF_MODE_SOLVE) return fail(PF_EINVAL, "pf_solve_noalloc runs PF_MODE_SOLVE only");
+  PfCav c{};
+  fill_params(c, params, precision);
+  c.blk0 = 0;
+  // Layout inside the scratch array: descriptor, two state parities, the
+  // active counter and the workgroup table, then the cavity's vectors and
+  // its partials. The right-hand side and the solution are the caller's:
+  // b is only read in PF_MODE_SOLVE, and the FP64 solution (x64, or x in
+  // FP64) is written straight into x.
+  const size_t n = (size_t)c.nx * c.ny;
+  const size_t vb = align256(n * esize(precision));
+  const size_t off_cav = 0;
+  const size_t off_scal = off_cav + align256(sizeof(PfCav));
+  const size_t off_active = off_scal + align256(2 * sizeof(PfScal));
+  const size_t off_blk = off_active + 256;
+  const size_t off_vec = off_blk + align256(sizeof(int) * (size_t)c.nblk);
+  const int nvec = (precision == PF_PREC_MIXED) ? 6 : 5;  // p r r1 d0 d1 (+ x: float32 inner iterate)
+  const size_t off_part = off_vec + (size_t)nvec * vb;
+  const size_t need = off_part + align256((size_t)2 * c.nblk * NP * sizeof(double));
+  if (need > PF_NOALLOC_SCRATCH_BYTES)
+    return fail(PF_EINVAL, "a " + std::to_string(c.nx) + "x" + std::to_string(c.ny) + " solve needs " +
+                               std::to_string(need) + " bytes of scratch; pf_solve_noalloc holds at most " +
+                               std::to_string(PF_NOALLOC_SCRATCH_BYTES) + " (PF_NOALLOC_SCRATCH_BYTES)");
+  char* base = nullptr;
+  rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  if (precision == PF_PREC_MIXED) {
+    rc = allow_region_lds();
+    if (rc) return rc;
+  }
+  char* w = base + off_vec;
+  c.p = w;
+  c.r = w + vb;
+  c.r1 = w + 2 * vb;
+  c.d0 = w + 3 * vb;
+  c.d1 = w + 4 * vb;
+  c.u = c.v = c.us = c.vs = c.p;  // not touched by the solve's phases
+  c.b = const_cast<double*>(rhs);
+  if (precision == PF_PREC_MIXED) {
+    c.x = w + 5 * vb;
+    c.x64 = x;
+  } else {
+    c.x = x;
+    c.x64 = x;
+  }
+  c.part = (double*)(base + off_part);
+  PfCav* dcav = (PfCav*)(base + off_cav);
+  PfScal* dscal = (PfScal*)(base + off_scal);
+  int* dactive = (int*)(base + off_active);
+  int* dblk = (int*)(base + off_blk);
+  PfScal s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.phase = PH_PINIT;
+  s0.pending = PH_IDLE;
+  s0.steps_left = 1;
+  s0.dt_prev = 1e-2;
+  hipStream_t st = (hipStream_t)stream;
+  // Descriptor, states, counter and table are written by a kernel from its
+  // arguments: no host staging buffer, no memset.
+  k_noalloc_init<<<dim3((unsigned)((c.nblk + 255) / 256)), dim3(256), 0, st>>>(c, s0, dcav, dscal, dactive, dblk,
+                                                                              c.nblk);
+  HIPCHK(hipGetLastError());
+  // The kernel instance a one-cavity batch of this cavity would run.
+  pf_batch tb;
+  tb.prec = precision;
+  tb.total_blocks = c.nblk;
+  tb.single_slot = 0;
+  tb.max_tpb = c.tpb;
+  tb.lds_floats = c.ca ? ca_lds_floats(c.tpb) : region_lds_floats(c.tpb);
+  tb.any_xd = c.xd != 0;
+  tb.xd_kinds = c.xd ? (c.ca ? 2 : 1) : 0;
+  tb.multi = c.rpw > 1;
+  tb.occ2 = (precision == PF_PREC_MIXED) && env_switch("PYFLOW_OCC2", 1) != 0 && c.ca && !c.xd && c.tpb <= 2 &&
+            c.rpw == 1;
+  tb.unroll = env_int("PYFLOW_UNROLL", 2);
+  const RoundLaunch L = round_launch(&tb);
+  KArgs K;
+  std::memset(&K, 0, sizeof(K));
+  K.cavs = dcav;
+  K.scal = dscal;
+  K.blk_cav = dblk;
+  K.active = dactive;
+  K.max_cav = 1;
+  K.slot0 = 0;
+  K.cav0 = c;
+  const int64_t guard = 4 * (int64_t)std::max(c.maxiter, 0) + 144;
+  int64_t round = 0, chunk_len = 64;
+  int host_active = 1;
+  while (true) {
+    for (int64_t k = 0; k < chunk_len; ++k, ++round) {
+      K.parity = (int)(round & 1);
+      void* args[] = {&K};
+      HIPCHK(hipLaunchKernel(L.fn, L.grid, L.blk, args, L.lds, st));
+    }
+    HIPCHK(hipMemcpyAsync(&host_active, dactive, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (host_active <= 0) break;
+    if (round > guard) return fail(PF_ESTATE, "round limit exceeded (internal error)");
+    chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
+  }
+  if (diag) {
+    PfScal fin;
+    HIPCHK(hipMemcpyAsync(&fin, dscal + ((round + 1) & 1), sizeof(PfScal), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    diag_from(fin, diag);
+  }
   return PF_OK;
 }
 

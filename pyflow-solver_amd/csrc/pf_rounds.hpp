@@ -1921,6 +1921,20 @@ __global__ void k_op(int opk, int nx, int ny, double dx, double dy, const double
   }
 }
 
+// State set-up of the allocation-free solve: descriptor, both state
+// parities, the active counter and the workgroup -> cavity table.
+__global__ void k_noalloc_init(const PfCav c, const PfScal s0, PfCav* __restrict__ cav, PfScal* __restrict__ scal,
+                               int* __restrict__ active, int* __restrict__ blk, int nblk) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    cav[0] = c;
+    scal[0] = s0;
+    scal[1] = s0;
+    active[0] = 1;
+  }
+  if (t < nblk) blk[t] = 0;
+}
+
 // One column of a cavity field as float64 (the driver's centerline).
 template <class T>
 __global__ void k_column(const T* __restrict__ src, int nx, int ny, int col, double* __restrict__ out) {

@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "pf_batch_set_dt", "pf_batch_run", "pf_batch_launch", "pf_batch_poll", "pf_batch_diag",
     "pf_batch_centerline", "pf_batch_rounds", "pf_batch_set_timing", "pf_batch_timing", "pf_op_apply",
     "pf_batch_start", "pf_batch_advance", "pf_batch_finished", "pf_batch_set_graph",
-    "pf_batch_resolved_params",
+    "pf_batch_resolved_params", "pf_solve_noalloc",
 )
 
 
@@ -124,6 +124,7 @@ def _declare(L):
         "pf_batch_start": ([P, I, I], I),
         "pf_batch_advance": ([P, I64], I),
         "pf_batch_finished": ([P, P], I),
+        "pf_solve_noalloc": ([I, C.POINTER(CavityParams), P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

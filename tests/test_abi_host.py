@@ -352,3 +352,22 @@ def test_checkpoint_round_trips_restart_meta(tmp_path):
     assert (q.tile_rows, q.max_workgroups, q.pcg_per_round, q.iterate64) == (0, 0, 0, 0)
     with pytest.raises(ValueError):
         _pin(loaded, cavity_params(21, 18, dx=1.0, dy=1.0), H.PREC_MIXED)
+
+
+def test_noalloc_solve_rejects_oversized_and_wrong_mode_without_gpu():
+    """pf_solve_noalloc checks its arguments before touching the device: a
+    cavity whose buffers exceed the static scratch array, or a mode other than
+    PF_MODE_SOLVE, is refused with PF_EINVAL and a message."""
+    import ctypes
+    import pyflow._hip as H
+    from pyflow._device import cavity_params
+    L = H.lib()
+    dummy = ctypes.c_void_p(0x1000)  # never dereferenced: the checks come first
+    p = cavity_params(4096, 4096, dx=1.0, dy=1.0, op=H.OP_NEG_LAP_PINNED, mode=H.MODE_SOLVE)
+    assert L.pf_solve_noalloc(H.PREC_MIXED, ctypes.byref(p), dummy, dummy, None, None) == -1
+    msg = L.pf_last_error()
+    assert b"4096x4096" in msg and b"PF_NOALLOC_SCRATCH_BYTES" in msg and str(64 << 20).encode() in msg
+    q = cavity_params(64, 64, dx=1.0, dy=1.0, mode=H.MODE_STEP)
+    assert L.pf_solve_noalloc(H.PREC_FP64, ctypes.byref(q), dummy, dummy, None, None) == -1
+    assert b"PF_MODE_SOLVE" in L.pf_last_error()
+    assert L.pf_solve_noalloc(H.PREC_FP64, ctypes.byref(q), None, dummy, None, None) == -1
