@@ -27,6 +27,20 @@
 
 namespace pf {
 
+// Stores of the PCG rounds' r, p and x: streaming (non-temporal) by default;
+// build with -DPF_STREAM_STORES=0 for ordinary stores.
+#ifndef PF_STREAM_STORES
+#define PF_STREAM_STORES 1
+#endif
+template <class V>
+__device__ __attribute__((always_inline)) inline void pcg_store(V v, V* p) {
+#if PF_STREAM_STORES
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 constexpr int TX = 64;        // tile width = wavefront width
 constexpr int TY = 16;        // tile height; one wave per tile row
 constexpr int NTH = TX * TY;  // threads per workgroup
@@ -1017,11 +1031,11 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
       // Streaming stores: these lines are next read in the next round, by
       // this and neighbouring workgroups, so keeping them dirty in L2 only
       // adds to the write-back at the end of the kernel.
-      __builtin_nontemporal_store(pk, &Pn[k]);
+      pcg_store(pk, &Pn[k]);
       if constexpr (XD)
-        __builtin_nontemporal_store(L.xo[m] + A.alpha * (double)pk, &C.x64[k]);
+        pcg_store(L.xo[m] + A.alpha * (double)pk, &C.x64[k]);
       else
-        __builtin_nontemporal_store(L.xo[m] + al * pk, &X[k]);
+        pcg_store(L.xo[m] + al * pk, &X[k]);
     }
     lp[(TY * m + lj + 2) * HP + li + 2] = pk;
     pkr[m] = pk;
@@ -1058,7 +1072,7 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
         acc[4] += (double)uk1[m] * (double)sv;
         acc[5] += (double)rk1[m] * (double)pkr[m];
       }
-      __builtin_nontemporal_store(rk1[m], &Rn[(size_t)j * nx + i]);
+      pcg_store(rk1[m], &Rn[(size_t)j * nx + i]);
     }
     lu[row * HP + li + 2] = uk1[m];
   }
@@ -1334,13 +1348,13 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
       const CaCell v = ca_steps(A, mo[m], ro[m], po[m], ao[m], cv, ev);
       const size_t k = (size_t)(j0 + TY * m + lj) * nx + i;
       // Streaming stores: the lines are next read in the next round.
-      __builtin_nontemporal_store(v.p2, &Pn[k]);
+      pcg_store(v.p2, &Pn[k]);
       if (!init) {
-        __builtin_nontemporal_store(v.r2, &Rn[k]);
+        pcg_store(v.r2, &Rn[k]);
         if constexpr (XD)
-          __builtin_nontemporal_store((xo[m] + A.ca0 * (double)po[m]) + A.ca1 * (double)v.p1, &C.x64[k]);
+          pcg_store((xo[m] + A.ca0 * (double)po[m]) + A.ca1 * (double)v.p1, &C.x64[k]);
         else
-          __builtin_nontemporal_store((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1, &X[k]);
+          pcg_store((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1, &X[k]);
       }
       r2o[m] = v.r2;
       put(lP2, o, cc[m], v.p2);
