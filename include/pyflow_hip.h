@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 1
+#define PF_ABI_VERSION 2
 
 #define PF_OK 0
 #define PF_EINVAL (-1)
@@ -89,6 +89,15 @@ typedef struct pf_cavity_params {
   int32_t max_workgroups;  /* cap on the cavity's workgroups (regions are grouped to fit) */
   int32_t pcg_per_round;   /* MIXED: PCG iterations per round, 1 or 2 (s-step, s = 2) */
   int32_t iterate64;       /* MIXED: PCG iterate in 1 = FP32, 2 = FP64; 0 = FP64 from 1536 cells a side */
+  /* MIXED, PF_OP_NEG_LAP_PINNED: after each solve, a Galerkin correction
+   * x += y (y.r)/(y.A y) along y = A^-1 1, which removes the near-constant
+   * error that FP32 rounding leaves along the pinned operator's slowest
+   * mode. y is solved once per cavity (tolerance 1e-2) before its first solve.
+   * 1 = off, 2 = on; 0 = on in PF_MODE_STEP and PF_MODE_PROJECT, off in
+   * PF_MODE_SOLVE (a one-off solve would pay the set-up solve each time).
+   * Ignored in FP64, which follows the reference's recurrence. */
+  int32_t deflate;
+  int32_t reserved;        /* 0 */
 } pf_cavity_params;
 
 /* Per-cavity results of the last completed step / projection / solve. */
@@ -150,7 +159,8 @@ int pf_batch_set_params(pf_batch* b, int cav, const pf_cavity_params* params);
 
 /* The parameters cavity `cav` runs with, every library default resolved:
  * tile_rows = its region height, max_workgroups = its workgroup count,
- * pcg_per_round (1 or 2) and iterate64 (1 or 2; 0 in FP64). Passing them back
+ * pcg_per_round (1 or 2), iterate64 (1 or 2; 0 in FP64) and deflate (1 or 2;
+ * 0 in FP64). Passing them back
  * (pf_batch_add / pf_batch_create) reproduces the same tiling and so the same
  * FP64 partial-sum order whatever PYFLOW_* environment the process has; the
  * checkpoint meta written at io/checkpoint.py:66-76 carries them so that a

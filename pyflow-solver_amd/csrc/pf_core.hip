@@ -74,6 +74,7 @@ struct pf_batch {
   std::vector<PfCav> hcav;       // slot -> description (nx == 0: free)
   std::vector<pf_cavity_params> hpar;  // slot -> the caller's parameters
   std::vector<char> used;
+  std::vector<char> dreset;  // slot -> its deflation vector is stale (operator or spacing changed)
   PfCav* dcav = nullptr;
   PfScal* dscal = nullptr;
   int* dtile = nullptr;  // workgroup -> cavity slot
@@ -129,6 +130,7 @@ static int check_params(const pf_cavity_params* p) {
   if (p->max_workgroups < 0) return fail(PF_EINVAL, "max_workgroups must be >= 0");
   if (p->pcg_per_round < 0 || p->pcg_per_round > 2) return fail(PF_EINVAL, "pcg_per_round must be 0, 1 or 2");
   if (p->iterate64 < 0 || p->iterate64 > 2) return fail(PF_EINVAL, "iterate64 must be 0, 1 or 2");
+  if (p->deflate < 0 || p->deflate > 2) return fail(PF_EINVAL, "deflate must be 0, 1 or 2");
   return PF_OK;
 }
 
@@ -181,6 +183,7 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   s += align256(n * sizeof(double));                           // b
   if (prec == PF_PREC_MIXED) s += align256(n * sizeof(double));  // x64
   s += align256((size_t)2 * nb * NP * sizeof(double));         // partials, two parities
+  if (prec == PF_PREC_MIXED) s += align256(n * sizeof(float));   // y (deflation)
   return s;
 }
 
@@ -232,7 +235,15 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   const int xd_dflt = xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0);
   c.xd = (prec == PF_PREC_MIXED) ? (p->iterate64 > 0 ? (p->iterate64 == 2 ? 1 : 0) : xd_dflt) : 0;
   choose_ca(c, prec, p->pcg_per_round);
+  const int defl_dflt = (p->mode != PF_MODE_SOLVE) ? 1 : 0;
+  c.defl = (prec == PF_PREC_MIXED && p->op == PF_OP_NEG_LAP_PINNED)
+               ? (p->deflate > 0 ? (p->deflate == 2 ? 1 : 0) : defl_dflt)
+               : 0;
 }
+
+// y = A^-1 1 depends on the operator and the spacing: a cavity whose
+// parameters change them solves it again before its next solve.
+static bool same_operator(const PfCav& a, const PfCav& b) { return a.op == b.op && a.dx == b.dx && a.dy == b.dy; }
 
 static void drop_graphs(pf_batch* b);
 
@@ -610,6 +621,7 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->unroll = env_int("PYFLOW_UNROLL", 2);
   b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 64));
   b->used.assign(max_cav, 0);
+  b->dreset.assign(max_cav, 0);
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dcav, sizeof(PfCav) * max_cav);
   if (e == hipSuccess) e = hipMalloc((void**)&b->dscal, 2 * sizeof(PfScal) * max_cav);
@@ -678,10 +690,13 @@ int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, s
     c.x64 = (double*)c.x;
   }
   c.part = (double*)w;
+  w += align256((size_t)2 * c.nblk * NP * sizeof(double));
+  c.y = (b->prec == PF_PREC_MIXED) ? (float*)w : nullptr;
   HIPCHK(hipMemsetAsync(workspace, 0, need, b->stream));
   b->hcav[s] = c;
   b->hpar[s] = *params;
   b->used[s] = 1;
+  b->dreset[s] = 0;
   PfScal& sc = b->hscal[s];
   std::memset(&sc, 0, sizeof(sc));
   sc.phase = PH_IDLE;
@@ -699,6 +714,7 @@ int pf_batch_remove(pf_batch* b, int slot) {
   b->used[slot] = 0;
   b->hcav[slot] = PfCav{};
   b->hpar[slot] = pf_cavity_params{};
+  b->dreset[slot] = 0;
   std::memset(&b->hscal[slot], 0, sizeof(PfScal));
   b->tables_dirty = true;
   return push_scal(b, slot);
@@ -776,6 +792,7 @@ int pf_batch_set_params(pf_batch* b, int slot, const pf_cavity_params* params) {
   nc.rpw = c.rpw;
   choose_ca(nc, b->prec, params->pcg_per_round);
   b->hpar[slot] = *params;
+  if (!same_operator(nc, c)) b->dreset[slot] = 1;
   // Unchanged parameters (the drop-in step loop sets them before every step)
   // leave the device tables and the instantiated graphs as they are.
   if (std::memcmp(&nc, &c, sizeof(PfCav)) != 0) {
@@ -795,6 +812,7 @@ int pf_batch_resolved_params(pf_batch* b, int slot, pf_cavity_params* out) {
   out->max_workgroups = c.nblk;
   out->pcg_per_round = (b->prec == PF_PREC_MIXED) ? (c.ca ? 2 : 1) : 1;
   out->iterate64 = (b->prec == PF_PREC_MIXED) ? (c.xd ? 2 : 1) : 0;
+  out->deflate = (b->prec == PF_PREC_MIXED) ? (c.defl ? 2 : 1) : 0;
   out->inner_rtol = c.inner_rtol;
   return PF_OK;
 }
@@ -910,6 +928,10 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
   for (int s = 0; s < b->max_cav; ++s) {
     PfScal& sc = b->hscal[s];
     if (!b->used[s]) continue;
+    if (b->dreset[s]) {
+      sc.dready = 0;
+      b->dreset[s] = 0;
+    }
     const bool on = !active || active[s];
     sc.pending = PH_IDLE;
     sc.done_now = 0;
@@ -922,6 +944,7 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
     sc.it_run = 0;
     sc.outer_run = 0;
     sc.rounds_run = 0;
+    sc.dsetup = 0;
     if (c.mode == PF_MODE_STEP) {
       sc.phase = PH_STATS;
       sc.steps_left = nsteps;
@@ -929,11 +952,12 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
       sc.phase = PH_PRED_A;
       sc.steps_left = 1;
     } else {
-      sc.phase = PH_PINIT;
+      enter_pinit(c, sc);
       sc.steps_left = 1;
     }
     const int64_t per = 4 * (int64_t)std::max(c.maxiter, 0) + 80;
-    guard = std::max(guard, (int64_t)sc.steps_left * per + 64);
+    const int64_t setup = c.defl ? 2 * (int64_t)std::max(c.maxiter, 0) + 80 : 0;  // deflation set-up solve
+    guard = std::max(guard, (int64_t)sc.steps_left * per + setup + 64);
     ++nact;
   }
   *b->hactive = nact;
@@ -1060,17 +1084,25 @@ int pf_batch_start(pf_batch* b, int slot, int nsteps) {
   PfScal& sc = b->hscal[slot];
   if (sc.phase != PH_IDLE && sc.phase != PH_DONE) return fail(PF_ESTATE, "cavity is already running");
   const PfCav& c = b->hcav[slot];
+  if (b->dreset[slot]) {
+    sc.dready = 0;
+    b->dreset[slot] = 0;
+  }
   sc.pending = PH_IDLE;
   sc.done_now = 0;
   sc.steps_done = 0;
   sc.it_run = 0;
   sc.outer_run = 0;
   sc.rounds_run = 0;
+  sc.dsetup = 0;
   if (c.mode == PF_MODE_STEP) {
     sc.phase = PH_STATS;
     sc.steps_left = nsteps;
+  } else if (c.mode == PF_MODE_PROJECT) {
+    sc.phase = PH_PRED_A;
+    sc.steps_left = 1;
   } else {
-    sc.phase = (c.mode == PF_MODE_PROJECT) ? PH_PRED_A : PH_PINIT;
+    enter_pinit(c, sc);
     sc.steps_left = 1;
   }
   rc = push_scal(b, slot);
@@ -1208,7 +1240,8 @@ int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double
   const size_t off_vec = off_blk + align256(sizeof(int) * (size_t)c.nblk);
   const int nvec = (precision == PF_PREC_MIXED) ? 6 : 5;  // p r r1 d0 d1 (+ x: float32 inner iterate)
   const size_t off_part = off_vec + (size_t)nvec * vb;
-  const size_t need = off_part + align256((size_t)2 * c.nblk * NP * sizeof(double));
+  const size_t off_y = off_part + align256((size_t)2 * c.nblk * NP * sizeof(double));
+  const size_t need = off_y + (c.defl ? align256(n * sizeof(float)) : 0);  // y: deflate = 2 only
   if (need > PF_NOALLOC_SCRATCH_BYTES)
     return fail(PF_EINVAL, "a " + std::to_string(c.nx) + "x" + std::to_string(c.ny) + " solve needs " +
                                std::to_string(need) + " bytes of scratch; pf_solve_noalloc holds at most " +
@@ -1236,13 +1269,14 @@ int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double
     c.x64 = x;
   }
   c.part = (double*)(base + off_part);
+  c.y = c.defl ? (float*)(base + off_y) : nullptr;  // solved again on every call
   PfCav* dcav = (PfCav*)(base + off_cav);
   PfScal* dscal = (PfScal*)(base + off_scal);
   int* dactive = (int*)(base + off_active);
   int* dblk = (int*)(base + off_blk);
   PfScal s0;
   std::memset(&s0, 0, sizeof(s0));
-  s0.phase = PH_PINIT;
+  enter_pinit(c, s0);
   s0.pending = PH_IDLE;
   s0.steps_left = 1;
   s0.dt_prev = 1e-2;
@@ -1275,7 +1309,7 @@ int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double
   K.max_cav = 1;
   K.slot0 = 0;
   K.cav0 = c;
-  const int64_t guard = 4 * (int64_t)std::max(c.maxiter, 0) + 144;
+  const int64_t guard = (c.defl ? 6 : 4) * (int64_t)std::max(c.maxiter, 0) + (c.defl ? 224 : 144);
   int64_t round = 0, chunk_len = 64;
   int host_active = 1;
   while (true) {

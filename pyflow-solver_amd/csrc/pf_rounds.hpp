@@ -66,7 +66,10 @@ enum Phase {
   PH_CG0 = 12,  // MIXED: w = A M r and the first dot products of an inner solve
   PH_CG = 13,   // MIXED: one single-reduction (Chronopoulos-Gear) PCG iteration, or two
                 // s-step iterations on a cavity with PfCav::ca
-  PH_DONE = 14,
+  PH_DEFL_A = 14,  // MIXED deflation: y.(b - A x) of the finished solve
+  PH_DEFL_B = 15,  // MIXED deflation: x += coef y
+  PH_DSTORE = 16,  // MIXED deflation set-up: y = float(x) of the solve of A y = 1, and y.A y
+  PH_DONE = 17,
 };
 
 // Partials per phase and which of them are NaN-propagating maxima.
@@ -83,6 +86,8 @@ __device__ __host__ inline int phase_nparts(int ph) {
     case PH_CORR_B: return 2;
     case PH_CG0: return 3;
     case PH_CG: return 3;
+    case PH_DEFL_A: return 1;
+    case PH_DSTORE: return 1;
     default: return 0;
   }
 }
@@ -103,6 +108,8 @@ struct PfCav {
   void* r1;                                     // MIXED: r double-buffered (r, r1); p in d0/d1
   double *b, *x64;                              // x64 aliases x in FP64 mode
   double* part;                                 // [2][NP][nblk]: slot k of workgroup b at k*nblk + b
+  float* y;                                     // MIXED: deflation vector (nullptr in FP64)
+  int defl;  // MIXED, pinned operator: Galerkin correction along y = A^-1 1 after each solve
 };
 
 // Scalar state of one cavity; two copies (round parity) per cavity.
@@ -112,7 +119,11 @@ struct PfScal {
   int steps_left, steps_done, it, it_total, info, outer, x_is_b;
   int hist_len, safeguard, nan, converged, it_run, outer_run, done_now;
   int ca_inc;      // two-iteration round: iterations the pending round runs (0: basis only)
-  int rounds_run;  // rounds this cavity executed a phase in since it was armed
+  int rounds_run;  // rounds this cavity executed a phase in since it was armed (deflation set-up excluded)
+  int dsetup;      // the running solve is the deflation set-up, A y = 1
+  int dready;      // y holds the deflation vector of this cavity's operator
+  double dden;     // y.A y of the stored y
+  double dcoef;    // Galerkin coefficient of the finished solve
   double dt_prev, dt, cfl_now, umax, vmax, hist[5], div_before;
   double mean, div_initial, bnorm, atol, inner_atol, rho, rho_prev, beta, pq, alpha, rnorm2, rn_outer;
   double rnorm_true, div_after, ru, rv, continuity;
@@ -215,6 +226,8 @@ __device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], doubl
     case PH_Q:                              \
     case PH_CORR_B: F(2, 0u); break;        \
     case PH_P:                              \
+    case PH_DEFL_A:                         \
+    case PH_DSTORE:                         \
     case PH_FIN: F(1, 0u); break;           \
     default: break;                         \
   }
@@ -234,6 +247,27 @@ __device__ inline void wave_reduce_phase(int ph, double (&v)[NP]) {
 // Scalar tails: run by one thread on the workgroup's shared copy of the state
 // after the pending phase's partials have been summed into tot[].
 // ---------------------------------------------------------------------------
+// Deflation (MIXED, pinned operator). The pinned Neumann operator has one
+// small isolated eigenvalue; its eigenvector is smooth, nearly constant away
+// from the pinned cell. FP32 rounding in the inner passes leaves an error
+// along it whose residual lies far below the tolerance, so no later pass
+// removes it, and the pressure carries a near-constant offset (2.7e-4
+// relative after the first step of the 1024^2 cavity, against 2.2e-6 for the
+// FP64 recurrence). After every solve the cavity takes the Galerkin step
+// x += y (y.r)/(y.A y) along y = A^-1 1, which minimises the A-norm error
+// along y and removes that offset. y is solved once per cavity, to kDeflTol:
+// the step is exact for whatever y is stored, and a y solved to 1e-1 gave the
+// same correction as one solved to 1e-12 on that cavity.
+constexpr double kDeflTol = 1e-2;
+
+// Enter PINIT. A deflating cavity without its y first solves A y = 1 (the
+// set-up, flagged in dsetup from its PINIT round on); PH_DSTORE stores y and
+// enters PINIT again for the solve the step needs.
+__device__ __host__ inline void enter_pinit(const PfCav& C, PfScal& S) {
+  S.dsetup = (C.defl && !S.dready) ? 1 : 0;
+  S.phase = PH_PINIT;
+}
+
 // First phase of a refinement pass: with an FP64 iterate there is no FP32
 // correction to fold in, so the pass starts with the FP64 residual.
 __device__ __attribute__((always_inline)) inline int outer_entry(const PfCav& C) { return C.xd ? PH_OUTER_B : PH_OUTER_A; }
@@ -425,12 +459,12 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
     case PH_PRED_B:
       S.mean = tot[0] / (double)((size_t)C.nx * C.ny);
       S.div_initial = sqrt(tot[1]);
-      S.phase = PH_PINIT;
+      enter_pinit(C, S);
       break;
     case PH_PINIT: {
       const double bn = sqrt(tot[0]);
       S.bnorm = bn;
-      S.atol = pymax(0.0, C.tol * bn);
+      S.atol = pymax(0.0, (S.dsetup ? kDeflTol : C.tol) * bn);
       S.it = 0;
       S.it_total = 0;
       S.outer = 0;
@@ -446,7 +480,7 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
       S.rn_outer = bn;
       if (C.maxiter <= 0) {  // the loop never runs: info = maxiter
         S.info = C.maxiter;
-        S.phase = PH_FIN;
+        S.phase = S.dsetup ? PH_DSTORE : PH_FIN;
         break;
       }
       if (C.prec == PF_PREC_MIXED) {
@@ -483,13 +517,16 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
       S.rn_outer = rn;
       S.outer += 1;
       S.outer_run += 1;
-      if (S.info != 0) {  // iteration budget exhausted
-        S.phase = PH_FIN;
-        break;
-      }
-      if (rn < S.atol) {
-        S.info = 0;
-        S.phase = PH_FIN;
+      // The solve ends when the FP64 residual meets the tolerance, also when
+      // the iteration budget ran out on this iterate (it then met the
+      // stopping rule within lin_maxiter iterations), or when the budget is
+      // spent.
+      if (rn < S.atol) S.info = 0;
+      if (rn < S.atol || S.info != 0) {
+        if (S.dsetup)
+          S.phase = PH_DSTORE;
+        else
+          S.phase = (C.defl && S.dready && S.dden > 0.0) ? PH_DEFL_A : PH_FIN;
         break;
       }
       S.it = 0;
@@ -511,6 +548,20 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
       }
       break;
     case PH_CG: tail_cg(C, S, tot); break;
+    case PH_DEFL_A: {
+      const double cf = tot[0] / S.dden;
+      S.dcoef = isfinite(cf) ? cf : 0.0;
+      S.phase = PH_DEFL_B;
+      break;
+    }
+    case PH_DEFL_B: S.phase = PH_FIN; break;
+    case PH_DSTORE:  // the set-up's iterations and passes are not the step's
+      S.dden = tot[0];
+      S.dready = 1;
+      S.it_run -= S.it_total;
+      S.outer_run -= S.outer;
+      enter_pinit(C, S);
+      break;
     case PH_FIN: {
       const double rn = sqrt(tot[0]);
       S.rnorm_true = rn;
@@ -548,11 +599,12 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
 
 // Scalars a phase body reads, hoisted out of the shared state.
 struct PhaseArgs {
-  double dt, mean, beta, alpha, x0;
+  double dt, mean, beta, alpha, x0;  // x0: x[0] in FIN and CORR_A; the Galerkin coefficient in DEFL_B
   int it, x_is_b, par;  // par = round parity
   float cx, cy, betaf, alphaf, minvf;  // float32 path: 1/dx^2, 1/dy^2, CG scalars, Jacobi weight
   float ca0f, cb0f, ca1f, cb1f;        // two-iteration round: alpha_k, beta_k, alpha_k+1, beta_k+1
   int ca_init;                         // two-iteration round that only forms the starting basis
+  int dset;                            // PINIT / OUTER_B of the deflation set-up: the right-hand side is 1
   double ca0, ca1;                     // alpha_k, alpha_k+1 for an FP64 iterate
 };
 
@@ -604,11 +656,16 @@ __device__ inline void cell_pred_b(const PfCav& C, const PhaseArgs& A, int j, in
 template <class T>
 __device__ inline void cell_pinit(const PfCav& C, const PhaseArgs& A, int j, int i, double* acc) {
   const size_t k = (size_t)j * C.nx + i;
-  double bb = C.b[k];
-  if (C.mode != PF_MODE_SOLVE) {
-    bb = bb - A.mean;
-    if (k == 0) bb = 0.0;
-    C.b[k] = bb;
+  double bb;
+  if (A.dset) {  // deflation set-up: A y = 1 off the pinned cell
+    bb = (k == 0) ? 0.0 : 1.0;
+  } else {
+    bb = C.b[k];
+    if (C.mode != PF_MODE_SOLVE) {
+      bb = bb - A.mean;
+      if (k == 0) bb = 0.0;
+      C.b[k] = bb;
+    }
   }
   const T rr = (T)bb;
   ((T*)((C.prec == PF_PREC_MIXED && A.par) ? C.r1 : C.r))[k] = rr;  // MIXED: r[q] of round q
@@ -803,13 +860,40 @@ __device__ inline void cell_outer_b(const PfCav& C, const PhaseArgs& A, int j, i
   const size_t k = (size_t)j * C.nx + i;
   Grid<double> XG{C.x64, C.nx};
   const double ax = apply_op_at<double>(C.op, XG, C.nx, C.ny, j, i, C.dx, C.dy);
-  const double r64 = C.b[k] - ax;
+  const double bk = A.dset ? ((k == 0) ? 0.0 : 1.0) : C.b[k];
+  const double r64 = bk - ax;
   const T rt = (T)r64;
   ((T*)(A.par ? C.r1 : C.r))[k] = rt;
   const T z = (T)jacobi_minv(C.jacobi, C.invd, C.nx, C.ny, j, i) * rt;
   acc[0] += r64 * r64;
   acc[1] += (double)rt * (double)z;
   acc[2] += (double)rt * (double)rt;
+}
+
+// Deflation: y.(b - A x) of the finished solve's FP64 iterate.
+__device__ inline void cell_defl_a(const PfCav& C, int j, int i, double* acc) {
+  const size_t k = (size_t)j * C.nx + i;
+  Grid<double> XG{C.x64, C.nx};
+  const double ax = apply_op_at<double>(C.op, XG, C.nx, C.ny, j, i, C.dx, C.dy);
+  acc[0] += (double)C.y[k] * (C.b[k] - ax);
+}
+
+__device__ inline void cell_defl_b(const PfCav& C, const PhaseArgs& A, int j, int i) {
+  const size_t k = (size_t)j * C.nx + i;
+  C.x64[k] = C.x64[k] + A.x0 * (double)C.y[k];  // x0 carries the coefficient
+}
+
+// Deflation set-up: y = float(x) of the solve of A y = 1, and y.A y of the
+// stored (rounded) y, with the stencil applied in FP64.
+__device__ inline void cell_dstore(const PfCav& C, int j, int i, double* acc) {
+  const int nx = C.nx;
+  const size_t k = (size_t)j * nx + i;
+  const double* X = C.x64;
+  auto YG = [&](int jj, int ii) -> double { return (double)(float)X[(size_t)jj * nx + ii]; };
+  const double ay = apply_op_at<double>(C.op, YG, nx, C.ny, j, i, C.dx, C.dy);
+  const float yv = (float)X[k];
+  C.y[k] = yv;
+  acc[0] += (double)yv * ay;
 }
 
 template <class T>
@@ -1495,6 +1579,9 @@ __device__ inline void cell_phase(int phase, const PfCav& C, const PhaseArgs& A,
     case PH_CORR_A: cell_corr_a<T>(C, A, j, i, acc); break;
     case PH_CORR_B: cell_corr_b<T>(C, j, i, acc); break;
     case PH_CG0: cell_cg0<T>(C, A, j, i, acc); break;
+    case PH_DEFL_A: cell_defl_a(C, j, i, acc); break;
+    case PH_DEFL_B: cell_defl_b(C, A, j, i); break;
+    case PH_DSTORE: cell_dstore(C, j, i, acc); break;
     default: break;
   }
 }
@@ -1643,12 +1730,13 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   A.it = st.it;
   A.x_is_b = st.x_is_b;
   A.par = parity;
-  A.x0 = 0.0;  // FIN reads x[0] in round_cold
+  A.x0 = (phase == PH_DEFL_B) ? st.dcoef : 0.0;  // FIN reads x[0] in round_cold
   A.cx = C.cxf;
   A.cy = C.cyf;
   A.betaf = (float)A.beta;
   A.alphaf = (float)A.alpha;
   A.minvf = C.invdf;
+  A.dset = st.dsetup;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
   double* part_out = C.part + (size_t)parity * nblk * NP + b;
   // Workgroup b owns regions b*rpw .. b*rpw+rpw-1 (fewer at the cavity's end);
@@ -1685,7 +1773,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
     __syncthreads();
     if (tid == 0) {
       st.pending = phase;
-      st.rounds_run += 1;
+      if (!st.dsetup) st.rounds_run += 1;
     }
     __syncthreads();
     if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
@@ -1766,12 +1854,13 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   A.it = st.it;
   A.x_is_b = st.x_is_b;
   A.par = parity;
-  A.x0 = 0.0;  // FIN reads x[0] in round_cold
+  A.x0 = (phase == PH_DEFL_B) ? st.dcoef : 0.0;  // FIN reads x[0] in round_cold
   A.cx = C.cxf;
   A.cy = C.cyf;
   A.betaf = (float)A.beta;
   A.alphaf = (float)A.alpha;
   A.minvf = C.invdf;
+  A.dset = st.dsetup;
   A.ca0f = (float)st.ca0;
   A.cb0f = (float)st.cb0;
   A.ca1f = (float)st.ca1;
@@ -1797,7 +1886,7 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
     __syncthreads();
     if (tid == 0) {
       st.pending = phase;
-      st.rounds_run += 1;
+      if (!st.dsetup) st.rounds_run += 1;
     }
     __syncthreads();
     if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];

@@ -54,7 +54,7 @@ def cavity_params(nx: int, ny: int, *, dx: float, dy: float, scheme: str | int =
                   Re: float = 100.0, cfl_target: float = 0.5, cfl_growth: float = 1.05,
                   lin_tol: float = 1e-10, lin_maxiter: int = 400, jacobi: bool = True,
                   op: int = H.OP_NEG_LAP_PINNED, mode: int = H.MODE_STEP,
-                  inner_rtol: float = DEFAULT_INNER_RTOL) -> H.CavityParams:
+                  inner_rtol: float = DEFAULT_INNER_RTOL, deflate: int = 0) -> H.CavityParams:
     p = H.CavityParams()
     p.nx, p.ny = int(nx), int(ny)
     if disable_advection:
@@ -75,6 +75,7 @@ def cavity_params(nx: int, ny: int, *, dx: float, dy: float, scheme: str | int =
     p.lin_tol = float(lin_tol)
     p.inv_diag = _inv_diag(float(dx), float(dy))
     p.inner_rtol = float(inner_rtol)
+    p.deflate = int(deflate)
     return p
 
 

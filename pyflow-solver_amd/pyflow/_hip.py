@@ -57,6 +57,8 @@ class CavityParams(C.Structure):
         ("inv_diag", C.c_double), ("inner_rtol", C.c_double),
         ("tile_rows", C.c_int32), ("max_workgroups", C.c_int32), ("pcg_per_round", C.c_int32),
         ("iterate64", C.c_int32),
+        ("deflate", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -162,7 +164,7 @@ def lib():
             except OSError as e:  # pragma: no cover - environment specific
                 raise HipUnavailableError(f"cannot load {path}: {e}") from e
             _declare(L)
-            if L.pf_abi_version() != 1:
+            if L.pf_abi_version() != 2:
                 raise HipUnavailableError("libpyflow_hip ABI version mismatch")
             if L.pf_sizeof_params() != C.sizeof(CavityParams) or L.pf_sizeof_diag() != C.sizeof(StepDiag):
                 raise HipUnavailableError("libpyflow_hip struct layout mismatch")

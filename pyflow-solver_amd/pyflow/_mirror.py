@@ -23,7 +23,7 @@ __all__ = ["StateMirror", "mirror_for", "config_precision"]
 _DEVICE_FIELDS = ("u", "v", "p")
 
 
-_PINNED_INT = ("tile_rows", "max_workgroups", "pcg_per_round", "iterate64")
+_PINNED_INT = ("tile_rows", "max_workgroups", "pcg_per_round", "iterate64", "deflate")
 
 
 def _precision_name(prec: int) -> str:
@@ -48,7 +48,7 @@ def _pin(state, params: H.CavityParams, prec: int) -> None:
         raise ValueError(f"state.meta['kernel_config'] was recorded for a {kc['nx']}x{kc['ny']} grid, "
                          f"not {params.nx}x{params.ny}")
     for k in _PINNED_INT:
-        setattr(params, k, int(kc[k]))
+        setattr(params, k, int(kc.get(k, 0)))  # a key missing from an older record: the library default
     params.inner_rtol = float(kc["inner_rtol"])
 
 

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert set(declared) == set(H.EXPORTED_SYMBOLS)
-    assert L.pf_abi_version() == 1
+    assert L.pf_abi_version() == 2
     assert L.pf_sizeof_params() == ctypes.sizeof(H.CavityParams)
     assert L.pf_sizeof_diag() == ctypes.sizeof(H.StepDiag)
 
@@ -229,8 +229,9 @@ def test_residual_manager_and_state_api():
 
 def test_autotune_blocking_mirror_matches_library():
     """The tuner's Python mirror of the library's blocking gives the same
-    workgroup count as the library: a MIXED workspace is 12 cell vectors plus
-    two parities x 16 FP64 partials per workgroup (256 B each)."""
+    workgroup count as the library: a MIXED workspace is 10 FP32 and 2 FP64
+    cell vectors, two parities x 16 FP64 partials per workgroup, and the FP32
+    deflation vector (each block 256 B aligned)."""
     import ctypes
     import pyflow._hip as H
     from pyflow import autotune as A
@@ -248,7 +249,7 @@ def test_autotune_blocking_mirror_matches_library():
         assert L.pf_batch_workspace_bytes(H.PREC_MIXED, 1, ctypes.byref(p), ctypes.byref(n)) == 0
         tpb, regions, rpw, nblk = A.blocking(nx, ny, rows, cap)
         assert nblk <= cap and rpw * nblk >= regions and tpb <= rows
-        assert n.value == 10 * al(nx * ny * 4) + 2 * al(nx * ny * 8) + al(2 * nblk * 16 * 8), (nx, ny, rows, cap)
+        assert n.value == 11 * al(nx * ny * 4) + 2 * al(nx * ny * 8) + al(2 * nblk * 16 * 8), (nx, ny, rows, cap)
 
 
 def test_tile_fields_validated():
@@ -335,7 +336,7 @@ def test_checkpoint_round_trips_restart_meta(tmp_path):
     import pyflow._hip as H
     st = allocate_state(20, 18)
     kc = {"precision": "mixed", "nx": 20, "ny": 18, "tile_rows": 2, "max_workgroups": 3, "pcg_per_round": 1,
-          "iterate64": 2, "inner_rtol": 0.03}
+          "iterate64": 2, "deflate": 1, "inner_rtol": 0.03}
     st.meta.update(dt_prev=0.25, divergence_history=[1.0, 2.0, 3.5], kernel_config=kc)
     ck = str(tmp_path / "ck.npz")
     save_checkpoint(ck, st, 7, 0.0, SimpleNamespace(seed=None))
@@ -346,10 +347,18 @@ def test_checkpoint_round_trips_restart_meta(tmp_path):
     assert config_precision(SimpleNamespace(precision="fp64"), loaded) == H.PREC_FP64
     p = cavity_params(20, 18, dx=1.0, dy=1.0)
     _pin(loaded, p, H.PREC_MIXED)
-    assert (p.tile_rows, p.max_workgroups, p.pcg_per_round, p.iterate64, p.inner_rtol) == (2, 3, 1, 2, 0.03)
+    assert (p.tile_rows, p.max_workgroups, p.pcg_per_round, p.iterate64, p.deflate, p.inner_rtol) == \
+        (2, 3, 1, 2, 1, 0.03)
     q = cavity_params(20, 18, dx=1.0, dy=1.0)
     _pin(loaded, q, H.PREC_FP64)  # another precision asked for explicitly: nothing pinned
-    assert (q.tile_rows, q.max_workgroups, q.pcg_per_round, q.iterate64) == (0, 0, 0, 0)
+    assert (q.tile_rows, q.max_workgroups, q.pcg_per_round, q.iterate64, q.deflate) == (0, 0, 0, 0, 0)
+    older = dict(kc)
+    del older["deflate"]  # recorded before the field existed: the library default
+    loaded.meta["kernel_config"] = older
+    o = cavity_params(20, 18, dx=1.0, dy=1.0, deflate=2)
+    _pin(loaded, o, H.PREC_MIXED)
+    assert o.deflate == 0
+    loaded.meta["kernel_config"] = kc
     with pytest.raises(ValueError):
         _pin(loaded, cavity_params(21, 18, dx=1.0, dy=1.0), H.PREC_MIXED)
 
