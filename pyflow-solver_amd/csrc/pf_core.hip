@@ -89,6 +89,7 @@ struct pf_batch {
   int xd_kinds = 0;      // rounds of those cavities: bit 0 one PCG iteration, bit 1 two
   bool multi = false;    // some cavity has several regions per workgroup
   bool occ2 = false;     // every cavity fits the two-workgroups-per-CU kernel
+  bool s3 = false;       // every cavity runs three PCG iterations per round (k_round_ca3)
   bool tables_dirty = true;
   int graph_len = 0;                      // rounds per captured graph (even; 0: direct launches)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by parity of the first round
@@ -128,7 +129,7 @@ static int check_params(const pf_cavity_params* p) {
   if (p->tile_rows != 0 && p->tile_rows != 1 && p->tile_rows != 2 && p->tile_rows != 4 && p->tile_rows != 8)
     return fail(PF_EINVAL, "tile_rows must be 0, 1, 2, 4 or 8");
   if (p->max_workgroups < 0) return fail(PF_EINVAL, "max_workgroups must be >= 0");
-  if (p->pcg_per_round < 0 || p->pcg_per_round > 2) return fail(PF_EINVAL, "pcg_per_round must be 0, 1 or 2");
+  if (p->pcg_per_round < 0 || p->pcg_per_round > 3) return fail(PF_EINVAL, "pcg_per_round must be 0, 1, 2 or 3");
   if (p->iterate64 < 0 || p->iterate64 > 2) return fail(PF_EINVAL, "iterate64 must be 0, 1 or 2");
   if (p->deflate < 0 || p->deflate > 2) return fail(PF_EINVAL, "deflate must be 0, 1 or 2");
   return PF_OK;
@@ -182,7 +183,7 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
   s += 10 * align256(n * esize(prec));  // u v p us vs r d0 d1 x r1
   s += align256(n * sizeof(double));                           // b
   if (prec == PF_PREC_MIXED) s += align256(n * sizeof(double));  // x64
-  s += align256((size_t)2 * nb * NP * sizeof(double));         // partials, two parities
+  s += align256((size_t)2 * nb * NPS * sizeof(double));         // partials, two parities
   if (prec == PF_PREC_MIXED) s += align256(n * sizeof(float));   // y (deflation)
   return s;
 }
@@ -191,10 +192,21 @@ static size_t cavity_bytes(int prec, const pf_cavity_params* p) {
 // 4-cell frames fit in LDS), with an FP32 or an FP64 iterate. Default on;
 // PYFLOW_CA=0 (FP32 iterate) or PYFLOW_CA_XD=0 (FP64 iterate) turns it off,
 // and a cavity's pcg_per_round (1 or 2) overrides both.
+// MIXED: three PCG iterations per round (6-cell frames) on FP32-iterate
+// cavities with regions of 2 or 4 tiles; PYFLOW_S3=1 makes it the default there.
 static void choose_ca(PfCav& c, int prec, int pcg_per_round) {
   const int dflt = c.xd ? (env_switch("PYFLOW_CA_XD", 1) != 0) : (env_switch("PYFLOW_CA", 1) != 0);
-  const int two = pcg_per_round > 0 ? (pcg_per_round == 2) : dflt;
-  c.ca = (prec == PF_PREC_MIXED && c.tpb <= 4 && two) ? 1 : 0;
+  const bool s3_ok = prec == PF_PREC_MIXED && !c.xd && (c.tpb == 2 || c.tpb == 4);
+  const int s = pcg_per_round > 0 ? pcg_per_round : (dflt ? (env_switch("PYFLOW_S3", 0) != 0 ? 3 : 2) : 1);
+  if (s == 3 && s3_ok)
+    c.ca = 2;
+  else
+    c.ca = (prec == PF_PREC_MIXED && c.tpb <= 4 && s >= 2) ? 1 : 0;
+}
+
+// Dynamic LDS of a cavity's float32 PCG round.
+static int lds_floats_of(const PfCav& c) {
+  return c.ca == 2 ? ca3_lds_floats(c.tpb) : (c.ca ? ca_lds_floats(c.tpb) : region_lds_floats(c.tpb));
 }
 
 static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
@@ -273,15 +285,21 @@ static int sync_tables(pf_batch* b) {
   b->xd_kinds = 0;
   b->multi = false;
   b->occ2 = (b->prec == PF_PREC_MIXED) && env_switch("PYFLOW_OCC2", 1) != 0;
+  int n3 = 0;
+  for (int s = 0; s < b->max_cav; ++s)
+    if (b->used[s] && b->hcav[s].ca == 2) ++n3;
+  if (n3 > 0 && n3 < nused)
+    return fail(PF_EINVAL, "cavities with three PCG iterations per round (pcg_per_round 3) cannot share a batch "
+                           "with cavities that run one or two");
+  b->s3 = n3 > 0;
   for (int s = 0; s < b->max_cav; ++s)
     if (b->used[s]) {
       b->max_tpb = std::max(b->max_tpb, b->hcav[s].tpb);
-      const int tpb = b->hcav[s].tpb;
-      b->lds_floats = std::max(b->lds_floats, b->hcav[s].ca ? ca_lds_floats(tpb) : region_lds_floats(tpb));
+      b->lds_floats = std::max(b->lds_floats, lds_floats_of(b->hcav[s]));
       b->any_xd = b->any_xd || b->hcav[s].xd;
       if (b->hcav[s].xd) b->xd_kinds |= b->hcav[s].ca ? 2 : 1;
       const PfCav& c = b->hcav[s];
-      b->occ2 = b->occ2 && c.ca && !c.xd && c.tpb <= 2 && c.rpw == 1;
+      b->occ2 = b->occ2 && c.ca == 1 && !c.xd && c.tpb <= 2 && c.rpw == 1;
       b->multi = b->multi || b->hcav[s].rpw > 1;
     }
   HIPCHK(hipMemcpyAsync(b->dcav, b->hcav.data(), sizeof(PfCav) * b->max_cav, hipMemcpyHostToDevice, b->stream));
@@ -356,7 +374,11 @@ static RoundLaunch round_launch(const pf_batch* b) {
   L.blk = dim3(TX, TY);
   L.grid = dim3(b->total_blocks);
   if (b->prec == PF_PREC_MIXED) {
-    if (b->occ2)
+    if (b->s3) {
+      const bool one = b->single_slot >= 0;
+      L.fn = b->multi ? (one ? (const void*)k_round_ca3<true, true> : (const void*)k_round_ca3<false, true>)
+                      : (one ? (const void*)k_round_ca3<true, false> : (const void*)k_round_ca3<false, false>);
+    } else if (b->occ2)
       L.fn = b->single_slot >= 0 ? (const void*)k_round_occ2<true> : (const void*)k_round_occ2<false>;
     else
       L.fn = round_fn_u<float, 2>(b);  // U only shapes the FP64 P/Q loops
@@ -592,8 +614,10 @@ int pf_batch_workspace_bytes(int precision, int ncav, const pf_cavity_params* pa
 static int graph_len_of(int rounds) { return rounds <= 0 ? 0 : std::min(1024, std::max(2, rounds & ~1)); }
 
 static int allow_region_lds() {
-  const int bytes = std::max(region_lds_floats(8), ca_lds_floats(4)) * (int)sizeof(float);
-  const void* fns[12] = {
+  const int bytes = std::max({region_lds_floats(8), ca_lds_floats(4), ca3_lds_floats(4)}) * (int)sizeof(float);
+  const void* fns[16] = {
+      (const void*)k_round_ca3<true, false>, (const void*)k_round_ca3<false, false>,
+      (const void*)k_round_ca3<true, true>, (const void*)k_round_ca3<false, true>,
       (const void*)k_round_occ2<true>, (const void*)k_round_occ2<false>,
       (const void*)k_round<float, 2, true, 0, false>,  (const void*)k_round<float, 2, false, 0, false>,
       (const void*)k_round<float, 2, false, 1, false>, (const void*)k_round<float, 2, false, 2, false>,
@@ -690,7 +714,7 @@ int pf_batch_add(pf_batch* b, const pf_cavity_params* params, void* workspace, s
     c.x64 = (double*)c.x;
   }
   c.part = (double*)w;
-  w += align256((size_t)2 * c.nblk * NP * sizeof(double));
+  w += align256((size_t)2 * c.nblk * NPS * sizeof(double));
   c.y = (b->prec == PF_PREC_MIXED) ? (float*)w : nullptr;
   HIPCHK(hipMemsetAsync(workspace, 0, need, b->stream));
   b->hcav[s] = c;
@@ -810,7 +834,7 @@ int pf_batch_resolved_params(pf_batch* b, int slot, pf_cavity_params* out) {
   *out = b->hpar[slot];
   out->tile_rows = c.tpb;
   out->max_workgroups = c.nblk;
-  out->pcg_per_round = (b->prec == PF_PREC_MIXED) ? (c.ca ? 2 : 1) : 1;
+  out->pcg_per_round = (b->prec == PF_PREC_MIXED) ? c.ca + 1 : 1;
   out->iterate64 = (b->prec == PF_PREC_MIXED) ? (c.xd ? 2 : 1) : 0;
   out->deflate = (b->prec == PF_PREC_MIXED) ? (c.defl ? 2 : 1) : 0;
   out->inner_rtol = c.inner_rtol;
@@ -1240,7 +1264,7 @@ int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double
   const size_t off_vec = off_blk + align256(sizeof(int) * (size_t)c.nblk);
   const int nvec = (precision == PF_PREC_MIXED) ? 6 : 5;  // p r r1 d0 d1 (+ x: float32 inner iterate)
   const size_t off_part = off_vec + (size_t)nvec * vb;
-  const size_t off_y = off_part + align256((size_t)2 * c.nblk * NP * sizeof(double));
+  const size_t off_y = off_part + align256((size_t)2 * c.nblk * NPS * sizeof(double));
   const size_t need = off_y + (c.defl ? align256(n * sizeof(float)) : 0);  // y: deflate = 2 only
   if (need > PF_NOALLOC_SCRATCH_BYTES)
     return fail(PF_EINVAL, "a " + std::to_string(c.nx) + "x" + std::to_string(c.ny) + " solve needs " +
@@ -1292,11 +1316,12 @@ int pf_solve_noalloc(int precision, const pf_cavity_params* params, const double
   tb.total_blocks = c.nblk;
   tb.single_slot = 0;
   tb.max_tpb = c.tpb;
-  tb.lds_floats = c.ca ? ca_lds_floats(c.tpb) : region_lds_floats(c.tpb);
+  tb.lds_floats = lds_floats_of(c);
   tb.any_xd = c.xd != 0;
   tb.xd_kinds = c.xd ? (c.ca ? 2 : 1) : 0;
   tb.multi = c.rpw > 1;
-  tb.occ2 = (precision == PF_PREC_MIXED) && env_switch("PYFLOW_OCC2", 1) != 0 && c.ca && !c.xd && c.tpb <= 2 &&
+  tb.s3 = c.ca == 2;
+  tb.occ2 = (precision == PF_PREC_MIXED) && env_switch("PYFLOW_OCC2", 1) != 0 && c.ca == 1 && !c.xd && c.tpb <= 2 &&
             c.rpw == 1;
   tb.unroll = env_int("PYFLOW_UNROLL", 2);
   const RoundLaunch L = round_launch(&tb);

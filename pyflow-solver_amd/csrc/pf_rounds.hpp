@@ -45,10 +45,12 @@ constexpr int TX = 64;        // tile width = wavefront width
 constexpr int TY = 16;        // tile height; one wave per tile row
 constexpr int NTH = TX * TY;  // threads per workgroup
 constexpr int NWAVE = NTH / 64;
-constexpr int NP = 16;        // FP64 partial slots per workgroup (stride)
+constexpr int NP = 16;        // FP64 partial values a thread carries (register arrays)
+constexpr int NPS = 32;       // FP64 partial slots per workgroup in memory (stride)
 constexpr int KMAX = 3;       // slots of every phase except the PCG rounds below
 constexpr int KXD = 6;        // slots of the FP64-iterate PCG round
 constexpr int KCA = 14;       // slots of the two-iteration PCG round
+constexpr int KCA3 = 27;      // slots of the three-iteration PCG round
 
 enum Phase {
   PH_IDLE = 0,
@@ -101,13 +103,13 @@ struct PfCav {
   int nreg, rpw;  // regions; regions per workgroup (workgroup b owns regions b*rpw .. b*rpw+rpw-1)
   int scheme, lid_on, jacobi, maxiter, op, mode, prec;
   int xd;  // MIXED: the PCG iterate is accumulated in FP64 (x64) instead of FP32
-  int ca;  // MIXED: two PCG iterations per round (s-step basis, 4-cell halo)
+  int ca;  // MIXED s-step rounds: 0 one PCG iteration per round; 1 two (4-cell halo); 2 three (6-cell halo)
   float cxf, cyf, invdf;  // (float)(1/dx^2), (float)(1/dy^2), (float)invd, formed on the host
   double lid, dx, dy, nu, cfl, growth, tol, invd, inner_rtol;
   void *u, *v, *p, *us, *vs, *r, *d0, *d1, *x;  // element type T
   void* r1;                                     // MIXED: r double-buffered (r, r1); p in d0/d1
   double *b, *x64;                              // x64 aliases x in FP64 mode
-  double* part;                                 // [2][NP][nblk]: slot k of workgroup b at k*nblk + b
+  double* part;                                 // [2][NPS][nblk]: slot k of workgroup b at k*nblk + b
   float* y;                                     // MIXED: deflation vector (nullptr in FP64)
   int defl;  // MIXED, pinned operator: Galerkin correction along y = A^-1 1 after each solve
 };
@@ -128,6 +130,7 @@ struct PfScal {
   double mean, div_initial, bnorm, atol, inner_atol, rho, rho_prev, beta, pq, alpha, rnorm2, rn_outer;
   double rnorm_true, div_after, ru, rv, continuity;
   double ca0, cb0, ca1, cb1;  // two-iteration round: alpha_k, beta_k, alpha_k+1, beta_k+1
+  double ca2, cb2;            // three-iteration round: alpha_k+2, beta_k+2
 };
 static_assert(sizeof(PfScal) % 4 == 0, "PfScal copied as 32-bit words");
 
@@ -195,7 +198,7 @@ __device__ inline void wave_reduce_k(double (&v)[NP]) {
 // after one barrier thread k < K folds the NWAVE totals in wave order and
 // stores slot k to dst[k * stride].
 template <int K, unsigned MM>
-__device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], double* dst, int stride) {
+__device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NPS], double* dst, int stride) {
   const int tid = tid_flat();
   const int lane = tid & 63, wave = tid >> 6;
   wave_reduce_k<K, MM>(v);
@@ -232,7 +235,7 @@ __device__ inline void block_partials_k(double (&v)[NP], double (*sm)[NP], doubl
     default: break;                         \
   }
 
-__device__ inline void block_partials_phase(int ph, double (&v)[NP], double (*sm)[NP], double* dst, int stride) {
+__device__ inline void block_partials_phase(int ph, double (&v)[NP], double (*sm)[NPS], double* dst, int stride) {
 #define PF_BP(K, MM) block_partials_k<K, MM>(v, sm, dst, stride)
   PF_PHASE_LAYOUT(PF_BP)
 #undef PF_BP
@@ -294,7 +297,7 @@ __device__ inline void inner_start(const PfCav& C, PfScal& S) {
   if (C.ca) {
     S.phase = PH_CG;
     S.ca_inc = 0;
-    S.ca0 = S.cb0 = S.ca1 = S.cb1 = 0.0;
+    S.ca0 = S.cb0 = S.ca1 = S.cb1 = S.ca2 = S.cb2 = 0.0;
   } else {
     S.phase = PH_CG0;
   }
@@ -409,6 +412,125 @@ __device__ __attribute__((always_inline)) inline void tail_ca(const PfCav& C, Pf
   S.ca_inc = n;
   S.alpha = n == 2 ? a1 : a0;
   S.beta = n == 2 ? b1 : b0;
+  S.phase = PH_CG;
+}
+
+// Tail of a three-iteration round (s = 3). The pending round's basis is
+// Y = [p, Bp, B^2 p, B^3 p, z, Bz, B^2 z] (B = M A); t holds the entries of
+// W = Y^T M^-1 Y that the recurrence reads (all but W00 and W04, slot
+// w3_slot(i, j)) and |r|^2 (slot 26). In the coefficient space of Y, with T
+// the shift along each chain (B Y c = Y T c), three PCG iterations are
+//   delta_j = p_j.W.T p_j,  alpha_j = gamma_j / delta_j,
+//   z_j+1 = z_j - alpha_j T p_j,  gamma_j+1 = z_j+1.W.z_j+1,  beta_j = gamma_j+1 / gamma_j,
+//   p_j+1 = z_j+1 + beta_j p_j,
+// from p_0 = e0, z_0 = e4, gamma_0 = W44. The supports of p_j, T p_j and
+// z_j+1 are fixed (bit masks below), so each form sums only its nonzero
+// terms. An iteration that the budget leaves out, or whose step breaks down,
+// keeps alpha = beta = 0: the round then moves by the iterations before it
+// and restarts the direction, as the two-iteration round does.
+__host__ __device__ constexpr int w3_slot(int i, int j) {
+  return (i > j) ? w3_slot(j, i)
+                 : (i == 0 ? ((j == 0 || j == 4) ? -1 : (j < 4 ? j - 1 : j - 2))
+                           : (i == 1 ? 5 : i == 2 ? 11 : i == 3 ? 16 : i == 4 ? 20 : i == 5 ? 23 : 25) + (j - i));
+}
+static_assert(w3_slot(6, 6) == 25 && w3_slot(0, 6) == 4 && w3_slot(3, 4) == 17 && w3_slot(5, 1) == 9, "W slots");
+__host__ __device__ constexpr unsigned w3_shift_mask(unsigned m) { return ((m & 0x7u) << 1) | ((m & 0x30u) << 1); }
+
+// u.W.v over the supports SU, SV, by template recursion so that every slot
+// index is a compile-time constant (the tail reads tot[] at fixed offsets).
+template <unsigned SV, int I, int J = 0>
+__device__ __attribute__((always_inline)) inline double w3_row(const double* t, const double (&v)[7]) {
+  if constexpr (J == 7) {
+    return 0.0;
+  } else if constexpr (((SV >> J) & 1u) != 0) {
+    constexpr int k = w3_slot(I, J);
+    static_assert(k >= 0, "the form reads only formed entries of W");
+    return t[k] * v[J] + w3_row<SV, I, J + 1>(t, v);
+  } else {
+    return w3_row<SV, I, J + 1>(t, v);
+  }
+}
+template <unsigned SU, unsigned SV, int I = 0>
+__device__ __attribute__((always_inline)) inline double w3_form(const double* t, const double (&u)[7],
+                                                                const double (&v)[7]) {
+  if constexpr (I == 7) {
+    return 0.0;
+  } else if constexpr (((SU >> I) & 1u) != 0) {
+    return u[I] * w3_row<SV, I>(t, v) + w3_form<SU, SV, I + 1>(t, u, v);
+  } else {
+    return w3_form<SU, SV, I + 1>(t, u, v);
+  }
+}
+
+// One iteration in coefficient space; SP: support of p_j, SZ: of z_j+1.
+// Returns whether a further iteration may follow.
+template <unsigned SP, unsigned SZ>
+__device__ __attribute__((always_inline)) inline bool ca3_iter(const double* t, double (&p)[7], double (&z)[7],
+                                                               double& gam, double& al, double& be) {
+  static_assert((SP & 0x48u) == 0, "the basis has no B^4 p or B^3 z");
+  constexpr unsigned ST = w3_shift_mask(SP);
+  static_assert((SZ & ST) == ST, "T p_j lies in the support of z_j+1");
+  double tp[7] = {0.0, p[0], p[1], p[2], 0.0, p[4], p[5]};
+  const double a = gam / w3_form<SP, ST>(t, p, tp);
+  if (!(a != 0.0 && isfinite(a))) return false;
+  double zn[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) zn[i] = z[i] - a * tp[i];
+  const double g1 = w3_form<SZ, SZ>(t, zn, zn);
+  double b = g1 / gam;
+  if (!isfinite(b)) b = 0.0;
+  al = a;
+  be = b;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    p[i] = zn[i] + b * p[i];
+    z[i] = zn[i];
+  }
+  gam = g1;
+  return g1 > 0.0;
+}
+
+__device__ __attribute__((always_inline)) inline void tail_ca3(const PfCav& C, PfScal& S, const double* t) {
+  S.pending = PH_IDLE;
+  const int inc = S.ca_inc;
+  S.it += inc;
+  S.it_total += inc;
+  S.it_run += inc;
+  S.rnorm2 = t[26];
+  S.rho = t[w3_slot(4, 4)];
+  if (S.it_total >= C.maxiter) {
+    S.info = C.maxiter;
+    S.phase = outer_entry(C);
+    return;
+  }
+  if (sqrt(S.rnorm2) < S.inner_atol) {
+    S.phase = outer_entry(C);
+    return;
+  }
+  double al[3] = {0.0, 0.0, 0.0}, be[3] = {0.0, 0.0, 0.0};
+  double p[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double z[7] = {0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+  double gam = t[w3_slot(4, 4)];
+  const int budget = C.maxiter - S.it_total;
+  // A failed first step counts one iteration with alpha = 0 (no move).
+  int n = 1;
+  if (ca3_iter<0x01u, 0x12u>(t, p, z, gam, al[0], be[0]) && budget >= 2) {
+    const bool more = ca3_iter<0x13u, 0x36u>(t, p, z, gam, al[1], be[1]);
+    if (al[1] != 0.0) n = 2;
+    if (more && n == 2 && budget >= 3) {
+      ca3_iter<0x37u, 0x7Eu>(t, p, z, gam, al[2], be[2]);
+      if (al[2] != 0.0) n = 3;
+    }
+  }
+  S.ca0 = al[0];
+  S.cb0 = be[0];
+  S.ca1 = al[1];
+  S.cb1 = be[1];
+  S.ca2 = al[2];
+  S.cb2 = be[2];
+  S.ca_inc = n;
+  S.alpha = al[n - 1];
+  S.beta = be[n - 1];
   S.phase = PH_CG;
 }
 
@@ -603,6 +725,7 @@ struct PhaseArgs {
   int it, x_is_b, par;  // par = round parity
   float cx, cy, betaf, alphaf, minvf;  // float32 path: 1/dx^2, 1/dy^2, CG scalars, Jacobi weight
   float ca0f, cb0f, ca1f, cb1f;        // two-iteration round: alpha_k, beta_k, alpha_k+1, beta_k+1
+  float ca2f, cb2f;                    // three-iteration round: alpha_k+2, beta_k+2
   int ca_init;                         // two-iteration round that only forms the starting basis
   int dset;                            // PINIT / OUTER_B of the deflation set-up: the right-hand side is 1
   double ca0, ca1;                     // alpha_k, alpha_k+1 for an FP64 iterate
@@ -1211,7 +1334,11 @@ __device__ __attribute__((always_inline)) inline void region_cg_f32(const PfCav&
 // expressions as their owners compute them, so they carry the same values.
 // HBM traffic per cell is that of the one-iteration round (read r, p, x;
 // write r, p, x) for two iterations; the frame re-reads hit L2.
-constexpr int HQ = TX + 8;  // LDS row pitch: 64 columns + 4-cell frame each side
+// LDS row pitch: 64 columns + 4-cell frame each side, plus one pad column.
+// An odd pitch spreads a frame column (cells one pitch apart, one per lane)
+// over all LDS banks; with 72 its cells fell on 4 banks. 13.78 -> 13.62 us
+// per round at 1024^2.
+constexpr int HQ = TX + 8 + 1;
 __host__ __device__ constexpr int ca_lds_floats(int m) { return 5 * (TY * m + 8) * HQ; }
 // Cells at Chebyshev distance exactly d from a region of rh rows, and within D.
 __host__ __device__ constexpr int ca_ring_cells(int rh, int d) { return 2 * (TX + 2 * d) + 2 * (rh + 2 * d - 2); }
@@ -1493,6 +1620,383 @@ __device__ __attribute__((always_inline)) inline void region_ca(const PfCav& C, 
   }
 }
 
+// Three PCG iterations per round (cavities with PfCav::ca == 2; the tail is
+// tail_ca3), s-step PCG with s = 3 on the basis
+// Y = [p, Bp, B^2 p, B^3 p, z, Bz, B^2 z], B = M A. Regions carry a 6-cell
+// frame; four LDS buffers L0..L3 rotate through the passes:
+//   pass 1  loads r_k, p_k at the cells and the frame (distance <= 6), x at
+//           the cells; p -> L0, z = M r -> L1;
+//   pass 2  a = A p (distance <= 5), e = A z (<= 4); M a -> L2, M e -> L3;
+//   pass 3  c = A M a (<= 4), f = A M e (<= 3); M c -> L0;
+//   pass 4  g = A M c (<= 3) and the three steps (<= 3)
+//             p1 = (z - a0 M a) + b0 p,          A p1 = (e - a0 c) + b0 a,
+//             r2 = (r - a0 a) - a1 A p1,  z2 = M r2,  p2 = z2 + b1 p1,
+//             A M A p1 = (f - a0 g) + b0 c,
+//             A p2 = ((e - a0 c) - a1 A M A p1) + b1 A p1,
+//             r3 = r2 - a2 A p2,  z3 = M r3,  p3 = z3 + b2 p2,
+//             x3 = ((x + a0 p) + a1 p1) + a2 p2;
+//           stores r3, p3, x3; p3 -> L2, z3 -> L3;
+//   pass 5  a' = A p3 (<= 2), e' = A z3 (<= 1); M a' -> L0, M e' -> L1;
+//   pass 6  c' = A M a' (<= 1), f' = A M e' (cells); M c' -> L2;
+//   pass 7  g' = A M c' and, at the cells, the 26 entries of W that the
+//           tail reads plus |r3|^2 as FP64 partials.
+// As in the two-iteration round, frame cells are computed from the same
+// inputs with the same expressions as their owners, and each of r, p, x is
+// read and written once per round, now for three iterations.
+constexpr int FW3 = 6;              // frame width of the three-iteration round
+constexpr int HQ3 = TX + 2 * FW3 + 1;  // LDS row pitch, odd as HQ
+__host__ __device__ constexpr int ca3_lds_floats(int m) { return 4 * (TY * m + 2 * FW3) * HQ3; }
+
+// LDS row/column of frame cell q of a region with a frame of FW cells,
+// numbered ring by ring as ca_frame_cell does; region cell (jj, ii) sits at
+// row jj + FW, column ii + FW.
+template <int FW>
+__host__ __device__ inline void frame_cell_w(int rh, int q, int& row, int& col) {
+  int d = 1;
+  for (; d < FW; ++d) {
+    const int nd = ca_ring_cells(rh, d);
+    if (q < nd) break;
+    q -= nd;
+  }
+  const int w = TX + 2 * d, hs = rh + 2 * d - 2;
+  if (q < w) {
+    row = FW - d;
+    col = FW - d + q;
+  } else if (q < 2 * w) {
+    row = FW - 1 + rh + d;
+    col = FW - d + (q - w);
+  } else if (q < 2 * w + hs) {
+    col = FW - d;
+    row = FW + 1 - d + (q - 2 * w);
+  } else {
+    col = FW - 1 + TX + d;
+    row = FW + 1 - d + (q - 2 * w - hs);
+  }
+}
+
+// The three steps at one cell (pass 4), shared by cells and frame.
+struct Ca3Cell {
+  float p1, p2, r3, z3, p3;
+};
+__device__ __attribute__((always_inline)) inline Ca3Cell ca3_steps(const PhaseArgs& A, float m, float r, float p,
+                                                                   float a, float e, float c, float f, float g) {
+  Ca3Cell o;
+  const float z = m * r;
+  const float ma = m * a;
+  o.p1 = (z - A.ca0f * ma) + A.cb0f * p;
+  const float ap1 = (e - A.ca0f * c) + A.cb0f * a;
+  const float r2 = (r - A.ca0f * a) - A.ca1f * ap1;
+  const float z2 = m * r2;
+  o.p2 = z2 + A.cb1f * o.p1;
+  const float amap1 = (f - A.ca0f * g) + A.cb0f * c;
+  const float az1 = e - A.ca0f * c;
+  const float ap2 = (az1 - A.ca1f * amap1) + A.cb1f * ap1;
+  o.r3 = r2 - A.ca2f * ap2;
+  o.z3 = m * o.r3;
+  o.p3 = o.z3 + A.cb2f * o.p2;
+  return o;
+}
+
+template <int M>
+__device__ __attribute__((always_inline)) inline void region_ca3(const PfCav& C, const PhaseArgs& A, int r, double* acc,
+                                                                 float* __restrict__ lds) {
+  const int nx = C.nx, ny = C.ny;
+  constexpr int RH = TY * M;
+  constexpr int NR = RH + 2 * FW3;
+  constexpr int F1 = ca_frame_cells(RH, 1), F2 = ca_frame_cells(RH, 2), F3 = ca_frame_cells(RH, 3);
+  constexpr int F4 = ca_frame_cells(RH, 4), F5 = ca_frame_cells(RH, 5), F6 = ca_frame_cells(RH, 6);
+  constexpr int NS = (F6 + NTH - 1) / NTH;  // frame slots per thread
+  static_assert(F3 <= NTH && NS == 2, "slot 0 covers distance 3; two slots cover the frame");
+  static_assert(NR * HQ3 * 4 == ca3_lds_floats(M), "LDS layout");
+  static_assert(NR * HQ3 < (1 << 15), "LDS offsets fit 15 bits");
+  float* __restrict__ L0 = lds;
+  float* __restrict__ L1 = lds + NR * HQ3;
+  float* __restrict__ L2 = lds + 2 * NR * HQ3;
+  float* __restrict__ L3 = lds + 3 * NR * HQ3;
+  // Buffers follow the round parity q as in region_ca: read r[q], p[q^1],
+  // write r[q^1], p[q]; the basis-only round reads r[q^1], writes only p[q].
+  const bool init = A.ca_init != 0;
+  const float* __restrict__ R = mixed_r(C, init ? (A.par ^ 1) : A.par);
+  const float* __restrict__ Pold = mixed_p(C, A.par ^ 1);
+  float* __restrict__ Rn = mixed_r(C, A.par ^ 1);
+  float* __restrict__ Pn = mixed_p(C, A.par);
+  float* __restrict__ X = (float*)C.x;
+  const int li = threadIdx.x, lj = threadIdx.y;
+  const int tid = lj * TX + li;
+  const int i0 = (r % C.tiles_x) * TX;
+  const int j0 = (r / C.tiles_x) * RH;
+  const int i = i0 + li;
+  // Operator without boundary tests, as in region_ca: boundary cells mirror
+  // their value into the LDS ghost across the boundary; bit 30 of a code
+  // marks a cell with ghosts, bit 31 an identity row.
+  const bool pinned = (C.op == 0);
+  const float scx = pinned ? -A.cx : A.cx, scy = pinned ? -A.cy : A.cy;
+  auto code_of = [&](int gj, int gi, int row, int col) -> unsigned {
+    const int o = row * HQ3 + col;
+    int g1 = o, g2 = o;
+    if (gi == nx - 1 && col + 1 < HQ3) g1 = o + 1;
+    if (gi == 0 && col >= 1) g1 = o - 1;
+    if (gj == ny - 1 && row + 1 < NR) g2 = o + HQ3;
+    if (gj == 0 && row >= 1) g2 = o - HQ3;
+    const bool id = pinned ? (gj == 0 && gi == 0) : (gj == 0 || gj == ny - 1 || gi == 0 || gi == nx - 1);
+    return (unsigned)g1 | ((unsigned)g2 << 15) | (id ? 0x80000000u : 0u) | ((g1 != o || g2 != o) ? 0x40000000u : 0u);
+  };
+  auto put = [&](float* L, int o, unsigned code, float v) {
+    L[o] = v;
+    if (code & 0x40000000u) {
+      L[code & 0x7fffu] = v;
+      L[(code >> 15) & 0x7fffu] = v;
+    }
+  };
+  auto opv = [&](const float* L, int o, unsigned code) -> float {
+    const float c = L[o];
+    const float t =
+        __fmul_rn((L[o + 1] - c) - (c - L[o - 1]), scx) + __fmul_rn((L[o + HQ3] - c) - (c - L[o - HQ3]), scy);
+    return (code & 0x80000000u) ? c : t;
+  };
+  // Pass 1: every global load of the thread, unconditional (out-of-range
+  // lanes read cell 0 and discard it).
+  float ro[M], po[M], xo[M];
+  bool in[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    in[m] = (i < nx) && (j < ny);
+    const size_t k = in[m] ? (size_t)j * nx + i : 0;
+    const float rv = R[k], pv = Pold[k], xv = X[k];
+    ro[m] = in[m] ? rv : 0.f;
+    po[m] = (in[m] && !init) ? pv : 0.f;
+    xo[m] = in[m] ? xv : 0.f;
+  }
+  float fr[NS], fp[NS], fm[NS];
+  int fo[NS];
+  unsigned fc[NS];
+  bool fin[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int q = tid + s * NTH;
+    int row = 0, col = 0;
+    if (q < F6) frame_cell_w<FW3>(RH, q, row, col);
+    fo[s] = row * HQ3 + col;
+    const int gj = j0 + row - FW3, gi = i0 + col - FW3;
+    fin[s] = (q < F6) && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+    fc[s] = fin[s] ? code_of(gj, gi, row, col) : 0;
+    const size_t kk = fin[s] ? (size_t)gj * nx + gi : 0;
+    const float rv = R[kk], pv = Pold[kk];
+    fr[s] = fin[s] ? rv : 0.f;
+    fp[s] = (fin[s] && !init) ? pv : 0.f;
+  }
+  const int o0 = (lj + FW3) * HQ3 + li + FW3;  // LDS offset of the thread's first cell; cell m at o0 + m*TY*HQ3
+  float mo[M];
+  unsigned cc[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int j = j0 + TY * m + lj;
+    mo[m] = in[m] ? minv_f32(C, A, j, i) : 0.f;
+    cc[m] = in[m] ? code_of(j, i, TY * m + lj + FW3, li + FW3) : 0;
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    fm[s] = 0.f;
+    if (fin[s]) {
+      const int row = fo[s] / HQ3, col = fo[s] - row * HQ3;
+      fm[s] = minv_f32(C, A, j0 + row - FW3, i0 + col - FW3);
+    }
+  }
+  // Per cell after pass 4: r3, p3, z3; after passes 5, 6: a', e', c', f'.
+  float r3o[M], p3o[M], z3o[M];
+  if (!init) {
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (in[m]) {
+        put(L0, o0 + m * TY * HQ3, cc[m], po[m]);
+        put(L1, o0 + m * TY * HQ3, cc[m], mo[m] * ro[m]);
+      }
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (fin[s]) {
+        put(L0, fo[s], fc[s], fp[s]);
+        put(L1, fo[s], fc[s], fm[s] * fr[s]);
+      }
+    __syncthreads();
+    // Pass 2: a = A p, e = A z.
+    float ao[M], eo[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      ao[m] = eo[m] = 0.f;
+      if (in[m]) {
+        const int o = o0 + m * TY * HQ3;
+        ao[m] = opv(L0, o, cc[m]);
+        eo[m] = opv(L1, o, cc[m]);
+        put(L2, o, cc[m], mo[m] * ao[m]);
+        put(L3, o, cc[m], mo[m] * eo[m]);
+      }
+    }
+    float fa = 0.f, fe = 0.f;  // slot 0 (distance <= 3 needs them in pass 4)
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int q = tid + s * NTH;
+      if (fin[s] && q < F5) {
+        const float av = opv(L0, fo[s], fc[s]);
+        put(L2, fo[s], fc[s], fm[s] * av);
+        if (s == 0) fa = av;
+      }
+      if (fin[s] && q < F4) {
+        const float ev = opv(L1, fo[s], fc[s]);
+        put(L3, fo[s], fc[s], fm[s] * ev);
+        if (s == 0) fe = ev;
+      }
+    }
+    __syncthreads();
+    // Pass 3: c = A M a, f = A M e.
+    float co[M], fo_[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      co[m] = fo_[m] = 0.f;
+      if (in[m]) {
+        const int o = o0 + m * TY * HQ3;
+        co[m] = opv(L2, o, cc[m]);
+        fo_[m] = opv(L3, o, cc[m]);
+        put(L0, o, cc[m], mo[m] * co[m]);
+      }
+    }
+    float fcv = 0.f, ffv = 0.f;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int q = tid + s * NTH;
+      if (fin[s] && q < F4) {
+        const float cv = opv(L2, fo[s], fc[s]);
+        put(L0, fo[s], fc[s], fm[s] * cv);
+        if (s == 0) fcv = cv;
+      }
+    }
+    if (fin[0] && tid < F3) ffv = opv(L3, fo[0], fc[0]);
+    __syncthreads();
+    // Pass 4: g = A M c and the three steps.
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      r3o[m] = p3o[m] = z3o[m] = 0.f;
+      if (in[m]) {
+        const int o = o0 + m * TY * HQ3;
+        const float gv = opv(L0, o, cc[m]);
+        const Ca3Cell v = ca3_steps(A, mo[m], ro[m], po[m], ao[m], eo[m], co[m], fo_[m], gv);
+        const size_t k = (size_t)(j0 + TY * m + lj) * nx + i;
+        // Streaming stores: the lines are next read in the next round.
+        pcg_store(v.p3, &Pn[k]);
+        pcg_store(v.r3, &Rn[k]);
+        pcg_store(((xo[m] + A.ca0f * po[m]) + A.ca1f * v.p1) + A.ca2f * v.p2, &X[k]);
+        r3o[m] = v.r3;
+        p3o[m] = v.p3;
+        z3o[m] = v.z3;
+        put(L2, o, cc[m], v.p3);
+        put(L3, o, cc[m], v.z3);
+      }
+    }
+    if (fin[0] && tid < F3) {
+      const float gv = opv(L0, fo[0], fc[0]);
+      const Ca3Cell v = ca3_steps(A, fm[0], fr[0], fp[0], fa, fe, fcv, ffv, gv);
+      put(L2, fo[0], fc[0], v.p3);
+      put(L3, fo[0], fc[0], v.z3);
+    }
+  } else {
+    // Basis-only round: p3 = z3 = M r, r3 = r; p is stored, r and x stay.
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      r3o[m] = p3o[m] = z3o[m] = 0.f;
+      if (in[m]) {
+        const int o = o0 + m * TY * HQ3;
+        const float zv = mo[m] * ro[m];
+        const size_t k = (size_t)(j0 + TY * m + lj) * nx + i;
+        pcg_store(zv, &Pn[k]);
+        r3o[m] = ro[m];
+        p3o[m] = zv;
+        z3o[m] = zv;
+        put(L2, o, cc[m], zv);
+        put(L3, o, cc[m], zv);
+      }
+    }
+    if (fin[0] && tid < F3) {
+      const float zv = fm[0] * fr[0];
+      put(L2, fo[0], fc[0], zv);
+      put(L3, fo[0], fc[0], zv);
+    }
+  }
+  __syncthreads();
+  // Pass 5: a' = A p3, e' = A z3.
+  float a2o[M], e2o[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    a2o[m] = e2o[m] = 0.f;
+    if (in[m]) {
+      const int o = o0 + m * TY * HQ3;
+      a2o[m] = opv(L2, o, cc[m]);
+      e2o[m] = opv(L3, o, cc[m]);
+      put(L0, o, cc[m], mo[m] * a2o[m]);
+      put(L1, o, cc[m], mo[m] * e2o[m]);
+    }
+  }
+  if (fin[0] && tid < F2) put(L0, fo[0], fc[0], fm[0] * opv(L2, fo[0], fc[0]));
+  if (fin[0] && tid < F1) put(L1, fo[0], fc[0], fm[0] * opv(L3, fo[0], fc[0]));
+  __syncthreads();
+  // Pass 6: c' = A M a', f' = A M e'.
+  float c2o[M], f2o[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    c2o[m] = f2o[m] = 0.f;
+    if (in[m]) {
+      const int o = o0 + m * TY * HQ3;
+      c2o[m] = opv(L0, o, cc[m]);
+      f2o[m] = opv(L1, o, cc[m]);
+      put(L2, o, cc[m], mo[m] * c2o[m]);
+    }
+  }
+  if (fin[0] && tid < F1) put(L2, fo[0], fc[0], fm[0] * opv(L0, fo[0], fc[0]));
+  __syncthreads();
+  // Pass 7: g' = A M c' and the partials at the cells. y = [p3, M a', M c',
+  // M g', z3, M e', M f'] and u = M^-1 y = [., a', c', g', r3, e', f']; the
+  // entry W_ij is y_i.u_j, with z3 standing in for M r3 beside a', c', g'.
+  // Products of two floats are exact in FP64, so each fma equals the
+  // product followed by the sum.
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    if (in[m]) {
+      const int o = o0 + m * TY * HQ3;
+      const float gv = opv(L2, o, cc[m]);
+      const float mv = mo[m];
+      const double p3 = p3o[m], z3 = z3o[m], r3 = r3o[m];
+      const double a = a2o[m], c = c2o[m], g = gv, e = e2o[m], f = f2o[m];
+      const double ma = mv * a2o[m], mc = mv * c2o[m], mg = mv * gv, me = mv * e2o[m], mf = mv * f2o[m];
+      acc[0] = __fma_rn(p3, a, acc[0]);     // W01
+      acc[1] = __fma_rn(p3, c, acc[1]);     // W02
+      acc[2] = __fma_rn(p3, g, acc[2]);     // W03
+      acc[3] = __fma_rn(p3, e, acc[3]);     // W05
+      acc[4] = __fma_rn(p3, f, acc[4]);     // W06
+      acc[5] = __fma_rn(ma, a, acc[5]);     // W11
+      acc[6] = __fma_rn(ma, c, acc[6]);     // W12
+      acc[7] = __fma_rn(ma, g, acc[7]);     // W13
+      acc[8] = __fma_rn(a, z3, acc[8]);     // W14
+      acc[9] = __fma_rn(ma, e, acc[9]);     // W15
+      acc[10] = __fma_rn(ma, f, acc[10]);   // W16
+      acc[11] = __fma_rn(mc, c, acc[11]);   // W22
+      acc[12] = __fma_rn(mc, g, acc[12]);   // W23
+      acc[13] = __fma_rn(c, z3, acc[13]);   // W24
+      acc[14] = __fma_rn(mc, e, acc[14]);   // W25
+      acc[15] = __fma_rn(mc, f, acc[15]);   // W26
+      acc[16] = __fma_rn(mg, g, acc[16]);   // W33
+      acc[17] = __fma_rn(g, z3, acc[17]);   // W34
+      acc[18] = __fma_rn(mg, e, acc[18]);   // W35
+      acc[19] = __fma_rn(mg, f, acc[19]);   // W36
+      acc[20] = __fma_rn(r3, z3, acc[20]);  // W44
+      acc[21] = __fma_rn(z3, e, acc[21]);   // W45
+      acc[22] = __fma_rn(z3, f, acc[22]);   // W46
+      acc[23] = __fma_rn(me, e, acc[23]);   // W55
+      acc[24] = __fma_rn(me, f, acc[24]);   // W56
+      acc[25] = __fma_rn(mf, f, acc[25]);   // W66
+      acc[26] = __fma_rn(r3, r3, acc[26]);  // |r3|^2
+    }
+  }
+}
+
 // Workgroup partials of the two-iteration round (KCA values per thread): a
 // transposed wave reduction. Each exchange pairs a lane with one whose index
 // differs in one more bit (lane distance 32 and 16 by the gfx950 permlane
@@ -1520,7 +2024,7 @@ __device__ __attribute__((always_inline)) inline double swap_sum(double x, doubl
   const double ys = __longlong_as_double((long long)((unsigned long long)lo[1] | ((unsigned long long)hi[1] << 32)));
   return xs + ys;
 }
-__device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], double* dst, int stride) {
+__device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NPS], double* dst, int stride) {
   static_assert(KCA <= 16 && NP >= 16, "16 slots carried through the exchange");
   const int tid = tid_flat();
   const int lane = tid & 63, wave = tid >> 6;
@@ -1557,6 +2061,59 @@ __device__ inline void block_partials_ca(double (&v)[NP], double (*sm)[NP], doub
   if ((lane & 3) == 0 && q < KCA) sm[wave][q] = d;
   __syncthreads();
   if (tid < KCA) {
+    double t = sm[0][tid];
+#pragma unroll
+    for (int w = 1; w < NWAVE; ++w) t += sm[w][tid];
+    dst[(size_t)tid * stride] = t;
+  }
+}
+
+// Workgroup partials of the three-iteration round: the transposed wave
+// reduction of block_partials_ca with 32 slots (27 used): exchanges at lane
+// distance 32 and 16 (permlane swaps), the row mirror, the half-row mirror
+// and quad_perm [2,3,0,1] halve the values carried, quad_perm [1,0,3,2]
+// finishes; lane 2q ends with value q.
+__device__ inline void block_partials_ca3(double (&v)[32], double (*sm)[NPS], double* dst, int stride) {
+  static_assert(KCA3 <= 32 && NPS >= 32, "32 slots carried through the exchange");
+  const int tid = tid_flat();
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int k = KCA3; k < 32; ++k) v[k] = 0.0;
+  double a[16], b[8], c[4], d[2];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) a[k] = swap_sum<true>(v[k], v[16 + k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) b[k] = swap_sum<false>(a[k], a[8 + k]);
+  {
+    const bool hi = lane & 8;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double keep = hi ? b[4 + k] : b[k];
+      const double give = hi ? b[k] : b[4 + k];
+      c[k] = keep + dpp_full<0x140>(give);  // row_mirror
+    }
+  }
+  {
+    const bool hi = lane & 4;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double keep = hi ? c[2 + k] : c[k];
+      const double give = hi ? c[k] : c[2 + k];
+      d[k] = keep + dpp_full<0x141>(give);  // row_half_mirror
+    }
+  }
+  double e;
+  {
+    const bool hi = lane & 2;
+    const double keep = hi ? d[1] : d[0];
+    const double give = hi ? d[0] : d[1];
+    e = keep + dpp_full<0x4E>(give);  // quad_perm [2,3,0,1]
+  }
+  e = e + dpp_full<0xB1>(e);  // quad_perm [1,0,3,2]
+  const int q = lane >> 1;
+  if ((lane & 1) == 0 && q < KCA3) sm[wave][q] = e;
+  __syncthreads();
+  if (tid < KCA3) {
     double t = sm[0][tid];
 #pragma unroll
     for (int w = 1; w < NWAVE; ++w) t += sm[w][tid];
@@ -1610,7 +2167,7 @@ struct KArgs {
 // copy passed by reference would have to be spilled to scratch first).
 template <class T, int V = 0>
 __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __restrict__ Cg, const PhaseArgs A0, int b,
-                                                     double (*sm)[NP], double* part_out, int pstride) {
+                                                     double (*sm)[NPS], double* part_out, int pstride) {
   const PfCav& C = *Cg;
   PhaseArgs A = A0;
   // FIN: p = x - x[0] (pressure_solver.py:147-149); the solution is b itself
@@ -1637,7 +2194,7 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
 // batches with more workgroups than CUs; they are issued in the PCG branch.)
 template <class T, int U, int M, bool XD, bool MULTI>
 __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c, int b, PfScal& st,
-                                  double (*sm)[NP], double* tot, float* lds) {
+                                  double (*sm)[NPS], double* tot, float* lds) {
   const int parity = KA.parity;
   const int tid = tid_flat();
   const uint32_t* src = (const uint32_t*)(KA.scal + (size_t)(parity ^ 1) * KA.max_cav + c);
@@ -1648,7 +2205,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   // lane l takes workgroups l, l+64, l+128, l+192) are loaded together, before
   // anything is known about the phase. Loads are unconditional (out-of-range
   // lanes read slot 0 and discard it).
-  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
+  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NPS;
   const int nblk = C.nblk;
   const int lane = tid & 63;
   // Folding waves: wave 0 takes slots 0..2; in the FP64-iterate kernel wave 1
@@ -1738,7 +2295,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
   A.minvf = C.invdf;
   A.dset = st.dsetup;
   double acc[NP] = {0.0, 0.0, 0.0, 0.0};
-  double* part_out = C.part + (size_t)parity * nblk * NP + b;
+  double* part_out = C.part + (size_t)parity * nblk * NPS + b;
   // Workgroup b owns regions b*rpw .. b*rpw+rpw-1 (fewer at the cavity's end);
   // without MULTI every cavity of the batch has rpw = 1.
   const int r0 = MULTI ? b * C.rpw : b;
@@ -1786,7 +2343,7 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
 // fourteen slots of the PCG round fold in parallel; thread 63 runs the tail.
 template <int M, bool MULTI, bool XD, int V = 0>
 __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c,
-                                                                    int b, PfScal& st, double (*sm)[NP], double* tot,
+                                                                    int b, PfScal& st, double (*sm)[NPS], double* tot,
                                                                     float* lds) {
   const int parity = KA.parity;
   const int tid = tid_flat();
@@ -1794,8 +2351,8 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
   uint32_t* dst = (uint32_t*)(KA.scal + (size_t)parity * KA.max_cav + c);
   constexpr int NW32 = (int)(sizeof(PfScal) / 4);
   static_assert(NW32 <= NTH, "state copied with one word per thread");
-  static_assert(NWAVE == NP && KCA <= NP, "one folding wave per partial slot");
-  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NP;
+  static_assert(KCA <= NWAVE, "one folding wave per partial slot");
+  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NPS;
   const int nblk = C.nblk;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int PRE = 4;
@@ -1878,9 +2435,134 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
       if (r > r0) __syncthreads();  // the previous region's LDS reads are done
       region_ca<M, XD>(C, A, r, acc, lds);
     }
-    block_partials_ca(acc, sm, C.part + (size_t)parity * C.nblk * NP + b, C.nblk);
+    block_partials_ca(acc, sm, C.part + (size_t)parity * C.nblk * NPS + b, C.nblk);
   } else {
-    round_cold<float, V>(phase, Cg, A, b, sm, C.part + (size_t)parity * C.nblk * NP + b, C.nblk);
+    round_cold<float, V>(phase, Cg, A, b, sm, C.part + (size_t)parity * C.nblk * NPS + b, C.nblk);
+  }
+  if (b == 0) {
+    __syncthreads();
+    if (tid == 0) {
+      st.pending = phase;
+      if (!st.dsetup) st.rounds_run += 1;
+    }
+    __syncthreads();
+    if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+  }
+}
+
+// Round body of a three-iteration cavity (PfCav::ca == 2): as round_body_ca,
+// with 27 partial slots (waves fold slots w and w + 16) and tail_ca3.
+template <int M, bool MULTI, int V = 0>
+__device__ __attribute__((always_inline)) inline void round_body_ca3(const KArgs& KA, const PfCav& C, const PfCav* Cg, int c,
+                                                                    int b, PfScal& st, double (*sm)[NPS], double* tot,
+                                                                    float* lds) {
+  const int parity = KA.parity;
+  const int tid = tid_flat();
+  const uint32_t* src = (const uint32_t*)(KA.scal + (size_t)(parity ^ 1) * KA.max_cav + c);
+  uint32_t* dst = (uint32_t*)(KA.scal + (size_t)parity * KA.max_cav + c);
+  constexpr int NW32 = (int)(sizeof(PfScal) / 4);
+  static_assert(NW32 <= NTH, "state copied with one word per thread");
+  static_assert(KCA3 <= 2 * NWAVE && NPS >= 2 * NWAVE, "two folding slots per wave");
+  const double* prev = C.part + (size_t)(parity ^ 1) * C.nblk * NPS;
+  const int nblk = C.nblk;
+  const int lane = tid & 63, wave = tid >> 6;
+  constexpr int PRE = 4;
+  double pre[PRE], pre2[PRE];  // slots wave and wave + NWAVE
+#pragma unroll
+  for (int t = 0; t < PRE; ++t) {
+    const int q = lane + 64 * t;
+    pre[t] = prev[(size_t)wave * nblk + (q < nblk ? q : 0)];
+    pre2[t] = prev[(size_t)(wave + NWAVE) * nblk + (q < nblk ? q : 0)];
+  }
+  const uint32_t sw = src[tid < NW32 ? tid : 0];
+  if (tid < NW32) ((uint32_t*)&st)[tid] = sw;
+  __syncthreads();
+  if (st.pending != PH_IDLE) {
+    const int pend = st.pending;
+    const bool hot = pend == PH_CG;
+    const int K = hot ? KCA3 : phase_nparts(pend);
+    if (wave < K) {
+      const bool mx = !hot && ((phase_maxmask(pend) >> wave) & 1u);
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < PRE; ++t)
+        if (lane + 64 * t < nblk) v = mx ? nanmax(v, pre[t]) : v + pre[t];
+      for (int q = lane + 64 * PRE; q < nblk; q += 64) {
+        const double w = prev[(size_t)wave * nblk + q];
+        v = mx ? nanmax(v, w) : v + w;
+      }
+      v = mx ? wave_reduce<true>(v) : wave_reduce<false>(v);
+      if (lane == 63) tot[wave] = v;
+    }
+    if (wave + NWAVE < K) {  // the PCG round only: sums
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < PRE; ++t)
+        if (lane + 64 * t < nblk) v = v + pre2[t];
+      for (int q = lane + 64 * PRE; q < nblk; q += 64) v = v + prev[(size_t)(wave + NWAVE) * nblk + q];
+      v = wave_reduce<false>(v);
+      if (lane == 63) tot[wave + NWAVE] = v;
+    }
+    __syncthreads();  // every slot's total visible to thread 63
+    if (tid == 63) {
+      if (__builtin_expect(hot, 1))
+        tail_ca3(C, st, tot);
+      else
+        tail<V>(*Cg, st, tot);
+    }
+    __syncthreads();  // tail result visible
+  }
+  const int phase = st.phase;
+  if (phase <= PH_IDLE || phase >= PH_DONE) {
+    if (b == 0) {
+      const int newly_done = st.done_now;
+      __syncthreads();
+      if (tid == 0) st.done_now = 0;
+      __syncthreads();
+      if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+      if (tid == 0 && newly_done) atomicSub(KA.active, 1);
+    }
+    return;
+  }
+  PhaseArgs A;
+  A.dt = st.dt;
+  A.mean = st.mean;
+  A.beta = st.beta;
+  A.alpha = st.alpha;
+  A.it = st.it;
+  A.x_is_b = st.x_is_b;
+  A.par = parity;
+  A.x0 = (phase == PH_DEFL_B) ? st.dcoef : 0.0;  // FIN reads x[0] in round_cold
+  A.cx = C.cxf;
+  A.cy = C.cyf;
+  A.betaf = (float)A.beta;
+  A.alphaf = (float)A.alpha;
+  A.minvf = C.invdf;
+  A.dset = st.dsetup;
+  A.ca0f = (float)st.ca0;
+  A.cb0f = (float)st.cb0;
+  A.ca1f = (float)st.ca1;
+  A.cb1f = (float)st.cb1;
+  A.ca2f = (float)st.ca2;
+  A.cb2f = (float)st.cb2;
+  A.ca_init = (st.ca_inc == 0);
+  A.ca0 = st.ca0;
+  A.ca1 = st.ca1;
+  double acc[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+  const int r0 = MULTI ? b * C.rpw : b;
+  const int r1 = MULTI ? min(r0 + C.rpw, C.nreg) : b + 1;
+  // The partials' address is formed after the region loop (from the
+  // descriptor again), which keeps it out of the loop's registers.
+  if (__builtin_expect(phase == PH_CG, 1)) {
+    for (int r = r0; r < r1; ++r) {
+      if (r > r0) __syncthreads();  // the previous region's LDS reads are done
+      region_ca3<M>(C, A, r, acc, lds);
+    }
+    block_partials_ca3(acc, sm, C.part + (size_t)parity * C.nblk * NPS + b, C.nblk);
+  } else {
+    round_cold<float, V>(phase, Cg, A, b, sm, C.part + (size_t)parity * C.nblk * NPS + b, C.nblk);
   }
   if (b == 0) {
     __syncthreads();
@@ -1900,9 +2582,9 @@ __device__ __attribute__((always_inline)) inline void round_body_ca(const KArgs&
 template <class T, int U, bool SINGLE, int XDK, bool MULTI>
 __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   __shared__ PfScal st;
-  __shared__ double sm[NWAVE][NP];
-  __shared__ double tot[NP];
-  extern __shared__ float lds_dyn[];  // float32 PCG region: region_lds_floats(tpb), or ca_lds_floats(tpb) floats
+  __shared__ double sm[NWAVE][NPS];
+  __shared__ double tot[NPS];
+  extern __shared__ float lds_dyn[];  // float32 PCG region: region_lds_floats, ca_lds_floats or ca3_lds_floats(tpb) floats
   const int g = blockIdx.x;
   int c;
   const PfCav* cp;
@@ -1957,6 +2639,37 @@ __global__ __launch_bounds__(NTH) void k_round(const KArgs KA) {
   }
 }
 
+// Batches whose cavities all run three PCG iterations per round (PfCav::ca
+// == 2, regions of 2 or 4 tiles, FP32 iterate). A kernel of its own, so
+// that neither its register allocation nor that of the other instances
+// pays for the other's code; the cold phases are their own instances (V = 2).
+template <bool SINGLE, bool MULTI>
+__global__ __launch_bounds__(NTH) void k_round_ca3(const KArgs KA) {
+  __shared__ PfScal st;
+  __shared__ double sm[NWAVE][NPS];
+  __shared__ double tot[NPS];
+  extern __shared__ float lds_dyn[];  // ca3_lds_floats(tpb) floats
+  const int g = blockIdx.x;
+  int c;
+  const PfCav* cp;
+  if constexpr (SINGLE) {
+    c = KA.slot0;
+    cp = &KA.cav0;
+  } else {
+    c = KA.blk_cav[g];
+    if (c < 0) return;
+    cp = KA.cavs + c;
+  }
+  const PfCav& C = *cp;
+  const PfCav* Cg = KA.cavs + c;
+  const int lg = g - C.blk0;
+  const int b = ((C.blk0 & 7) == 0 && (C.nblk & 7) == 0) ? (lg & 7) * (C.nblk >> 3) + (lg >> 3) : lg;
+  if (C.tpb == 2)
+    round_body_ca3<2, MULTI, 2>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
+  else
+    round_body_ca3<4, MULTI, 2>(KA, C, Cg, c, b, st, sm, tot, lds_dyn);
+}
+
 // Two workgroups per CU: batches whose cavities all run the two-iteration
 // FP32-iterate round on regions of at most two tiles, one region per
 // workgroup (LDS: 57.6 KB per workgroup). The register budget is 64 VGPRs
@@ -1967,8 +2680,8 @@ template <bool SINGLE>
 __global__ __attribute__((amdgpu_flat_work_group_size(NTH, NTH), amdgpu_waves_per_eu(8, 8))) void k_round_occ2(
     const KArgs KA) {
   __shared__ PfScal st;
-  __shared__ double sm[NWAVE][NP];
-  __shared__ double tot[NP];
+  __shared__ double sm[NWAVE][NPS];
+  __shared__ double tot[NPS];
   extern __shared__ float lds_dyn[];  // ca_lds_floats(tpb) floats
   const int g = blockIdx.x;
   int c;

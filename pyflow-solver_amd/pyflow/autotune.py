@@ -7,8 +7,8 @@ The PCG round dominates every workload (about 98% of the launches of a
 * ``tile_rows``: region height in 16-row tiles (regions are 64 cells wide);
 * ``max_workgroups``: cap on the cavity's workgroups; regions are grouped so
   that each workgroup walks ``rpw`` consecutive regions;
-* ``pcg_per_round``: MIXED, 1 or 2 PCG iterations per round (s-step PCG,
-  s = 2), with an FP32 or (on large grids) an FP64 iterate.
+* ``pcg_per_round``: MIXED, 1, 2 or 3 PCG iterations per round (s-step PCG,
+  s = 2 or 3), with an FP32 or (on large grids, s <= 2) an FP64 iterate.
 
 They change where cells are computed and the order in which FP64 partial sums
 are folded, never the arithmetic of a cell. ``tune`` times each distinct
@@ -73,19 +73,25 @@ def candidates(nx: int, ny: int, precision: str | int = "mixed",
     """Distinct (tile_rows, max_workgroups, pcg_per_round) settings for a
     cavity: settings that give the library the same blocking are listed once."""
     prec = precision_code(precision)
+    mixed = prec == H.PREC_MIXED
+    # The library keeps the PCG iterate in FP64 from 1536 cells a side (or as
+    # PYFLOW_XD says); three iterations per round need an FP32 iterate.
+    xd_env = os.environ.get("PYFLOW_XD")
+    xd = (xd_env not in ("0", "")) if xd_env is not None else max(nx, ny) >= 1536
     seen, out = set(), []
     for r in rows:
         for cap in caps:
             tpb, _, rpw, nblk = blocking(nx, ny, r, cap)
-            for s in (2, 1):
-                two = s == 2 and prec == H.PREC_MIXED and tpb <= 4
-                if s == 2 and not two:
+            for s in (3, 2, 1):
+                if s == 3 and not (mixed and tpb in (2, 4) and not xd):
                     continue
-                key = (tpb, rpw, nblk, two)
+                if s == 2 and not (mixed and tpb <= 4):
+                    continue
+                key = (tpb, rpw, nblk, s if mixed else 1)
                 if key in seen:
                     continue
                 seen.add(key)
-                out.append(Tile(tpb, cap, s if prec == H.PREC_MIXED else 1))
+                out.append(Tile(tpb, cap, s if mixed else 1))
     return out
 
 

@@ -230,8 +230,8 @@ def test_residual_manager_and_state_api():
 def test_autotune_blocking_mirror_matches_library():
     """The tuner's Python mirror of the library's blocking gives the same
     workgroup count as the library: a MIXED workspace is 10 FP32 and 2 FP64
-    cell vectors, two parities x 16 FP64 partials per workgroup, and the FP32
-    deflation vector (each block 256 B aligned)."""
+    cell vectors, two parities x 32 FP64 partial slots per workgroup, and the
+    FP32 deflation vector (each block 256 B aligned)."""
     import ctypes
     import pyflow._hip as H
     from pyflow import autotune as A
@@ -249,7 +249,7 @@ def test_autotune_blocking_mirror_matches_library():
         assert L.pf_batch_workspace_bytes(H.PREC_MIXED, 1, ctypes.byref(p), ctypes.byref(n)) == 0
         tpb, regions, rpw, nblk = A.blocking(nx, ny, rows, cap)
         assert nblk <= cap and rpw * nblk >= regions and tpb <= rows
-        assert n.value == 11 * al(nx * ny * 4) + 2 * al(nx * ny * 8) + al(2 * nblk * 16 * 8), (nx, ny, rows, cap)
+        assert n.value == 11 * al(nx * ny * 4) + 2 * al(nx * ny * 8) + al(2 * nblk * 32 * 8), (nx, ny, rows, cap)
 
 
 def test_tile_fields_validated():
@@ -258,7 +258,8 @@ def test_tile_fields_validated():
     from pyflow._device import cavity_params
     L = H.lib()
     n = ctypes.c_size_t()
-    for field, bad in (("tile_rows", 3), ("tile_rows", 16), ("max_workgroups", -1), ("pcg_per_round", 3)):
+    for field, bad in (("tile_rows", 3), ("tile_rows", 16), ("max_workgroups", -1), ("pcg_per_round", 4),
+                       ("pcg_per_round", -1), ("deflate", 3), ("iterate64", 3)):
         p = cavity_params(64, 64, dx=1.0, dy=1.0)
         setattr(p, field, bad)
         assert L.pf_batch_workspace_bytes(1, 1, ctypes.byref(p), ctypes.byref(n)) == -1
@@ -273,9 +274,13 @@ def test_autotune_candidates_are_distinct_and_valid():
         keys = {(A.blocking(n, n, c.tile_rows, c.max_workgroups), c.pcg_per_round) for c in cs}
         assert len(keys) == len(cs)
         for c in cs:
-            assert c.pcg_per_round in (1, 2)
+            tpb = A.blocking(n, n, c.tile_rows, c.max_workgroups)[0]
+            assert c.pcg_per_round in (1, 2, 3)
             if c.pcg_per_round == 2:  # s-step only where the library runs it
-                assert prec == "mixed" and A.blocking(n, n, c.tile_rows, c.max_workgroups)[0] <= 4
+                assert prec == "mixed" and tpb <= 4
+            if c.pcg_per_round == 3:  # FP32 iterate (below 1536 cells a side), regions of 2 or 4 tiles
+                assert prec == "mixed" and tpb in (2, 4) and n < 1536
+        assert (prec == "mixed" and n < 1536) == any(c.pcg_per_round == 3 for c in cs), (n, prec)
 
 
 def test_serving_footprint_and_arena():
