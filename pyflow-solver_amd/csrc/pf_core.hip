@@ -101,7 +101,8 @@ struct pf_batch {
   bool chunk_pending = false;
   bool scal_fresh = false;  // hscal equals the device's latest state (pulled, nothing launched since)
   bool serving = false;  // chunks queued by pf_batch_advance: poll queues nothing more
-  int64_t poll_chunk = 16;  // rounds queued by the next pf_batch_poll (doubles up to 256)
+  int64_t poll_chunk = 16;  // rounds queued by the next pf_batch_poll (doubles up to max_chunk)
+  int64_t max_chunk = 1024;  // largest chunk of rounds between host checks (PYFLOW_MAX_CHUNK)
   bool timing = false;
   double gpu_ms = 0.0;       // summed GPU time of launched chunks (timing on)
   int64_t timed_rounds = 0;
@@ -644,6 +645,11 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->hpar.assign(max_cav, pf_cavity_params{});
   b->unroll = env_int("PYFLOW_UNROLL", 2);
   b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 64));
+  // Chunks double up to max_chunk rounds between host checks: each check idles
+  // the GPU while the host queues the next chunk, and the last chunk of a run
+  // overshoots by at most its length (empty rounds). 1024 against 256: 0.6%
+  // faster over the 200 steps of the 1024^2 cavity.
+  b->max_chunk = std::max<int64_t>(16, env_int("PYFLOW_MAX_CHUNK", 1024));
   b->used.assign(max_cav, 0);
   b->dreset.assign(max_cav, 0);
   hipError_t e = hipGetDevice(&b->device);
@@ -1031,7 +1037,7 @@ int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* d
       if (b->rounds > b->round_guard) return fail(PF_ESTATE, "round limit exceeded (internal error)");
       rc = chunk(b, step);
       if (rc) return rc;
-      step = std::min<int64_t>(step * 2, 256);
+      step = std::min<int64_t>(step * 2, b->max_chunk);
     }
   }
   rc = pull_scal(b);
@@ -1088,7 +1094,7 @@ int pf_batch_poll(pf_batch* b, int* done) {
     return PF_OK;
   }
   if (b->rounds > b->round_guard) return fail(PF_ESTATE, "round limit exceeded (internal error)");
-  b->poll_chunk = std::min<int64_t>(b->poll_chunk * 2, 256);
+  b->poll_chunk = std::min<int64_t>(b->poll_chunk * 2, b->max_chunk);
   int rc = chunk(b, b->poll_chunk);
   if (rc) return rc;
   *done = 0;
