@@ -94,7 +94,12 @@ struct pf_batch {
   int graph_len = 0;                      // rounds per captured graph (even; 0: direct launches)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by parity of the first round
   int64_t rounds = 0;
-  int64_t predicted_rounds = 0;
+  int64_t predicted_rounds = 0;  // rounds the slowest cavity ran in the last run (first chunk of the next)
+  int64_t step_rounds = 0;       // tracked run: rounds per step the cavity ran in the last one
+  PfScal* hobs = nullptr;        // pinned: the tracked cavity's state after the pending chunk
+  int obs_slot = -1;             // cavity whose step ends size the chunks of this run, else -1
+  int64_t max_ahead = 4096;      // tracked run: largest chunk queued on a prediction
+  bool track = true;             // size the chunks of one stepping cavity from its step ends (PYFLOW_TRACK)
   int64_t round_guard = 0;
   hipEvent_t ev = nullptr;
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;  // timing of the pending chunk
@@ -650,6 +655,8 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   // overshoots by at most its length (empty rounds). 1024 against 256: 0.6%
   // faster over the 200 steps of the 1024^2 cavity.
   b->max_chunk = std::max<int64_t>(16, env_int("PYFLOW_MAX_CHUNK", 1024));
+  b->track = env_switch("PYFLOW_TRACK", 1) != 0;
+  b->max_ahead = std::max<int64_t>(16, env_int("PYFLOW_MAX_AHEAD", 4096));
   b->used.assign(max_cav, 0);
   b->dreset.assign(max_cav, 0);
   hipError_t e = hipGetDevice(&b->device);
@@ -659,6 +666,7 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   if (e == hipSuccess) e = hipMalloc((void**)&b->dactive, sizeof(int));
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->hscal, sizeof(PfScal) * max_cav, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->hactive, sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->hobs, sizeof(PfScal), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_t0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_t1);
@@ -795,6 +803,7 @@ int pf_batch_destroy(pf_batch* b) {
   if (b->dactive) (void)hipFree(b->dactive);
   if (b->hscal) (void)hipHostFree(b->hscal);
   if (b->hactive) (void)hipHostFree(b->hactive);
+  if (b->hobs) (void)hipHostFree(b->hobs);
   if (b->ev) (void)hipEventDestroy(b->ev);
   if (b->ev_t0) (void)hipEventDestroy(b->ev_t0);
   if (b->ev_t1) (void)hipEventDestroy(b->ev_t1);
@@ -953,6 +962,7 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
   if (nsteps <= 0) return fail(PF_EINVAL, "nsteps must be > 0");
   int rc = sync_tables(b);
   if (rc) return rc;
+  b->obs_slot = -1;
   int nact = 0;
   int64_t guard = 64;
   for (int s = 0; s < b->max_cav; ++s) {
@@ -974,6 +984,8 @@ static int arm(pf_batch* b, int nsteps, const int32_t* active) {
     sc.it_run = 0;
     sc.outer_run = 0;
     sc.rounds_run = 0;
+    sc.step_round = 0;
+    sc.step_len = 0;
     sc.dsetup = 0;
     if (c.mode == PF_MODE_STEP) {
       sc.phase = PH_STATS;
@@ -1011,18 +1023,52 @@ static int chunk(pf_batch* b, int64_t n) {
     b->timed_rounds += n;
   }
   HIPCHK(hipMemcpyAsync(b->hactive, b->dactive, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  if (b->obs_slot >= 0) {
+    const int par = (int)((b->round + 1) & 1);  // buffer of the last round
+    HIPCHK(hipMemcpyAsync(b->hobs, b->dscal + (size_t)par * b->max_cav + b->obs_slot, sizeof(PfScal),
+                          hipMemcpyDeviceToHost, b->stream));
+  }
   HIPCHK(hipEventRecord(b->ev, b->stream));
   b->chunk_pending = true;
   return PF_OK;
+}
+
+// Rounds to queue next in a tracked run (one cavity stepping nsteps > 1 steps),
+// from its state after the last chunk: the length of its last step gives the
+// rounds left, and the chunk queues half of them, so the run needs few host
+// checks and ends with few empty rounds even where step lengths drift (from
+// rest they change several-fold within a few steps). Before the first step
+// end, and once the prediction is used up, the chunks double from 16.
+static int64_t tracked_chunk(const pf_batch* b, int64_t& over, int nsteps) {
+  const PfScal& o = *b->hobs;
+  if (o.steps_done > 0) {
+    const int64_t left = (int64_t)(nsteps - o.steps_done) * o.step_len - (o.rounds_run - o.step_round);
+    if (left > 32) {
+      over = 0;
+      return std::min<int64_t>(b->max_ahead, left / 2);
+    }
+  }
+  over = over ? std::min<int64_t>(2 * over, b->max_chunk) : 16;
+  return over;
 }
 
 int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* diag) {
   if (b) b->serving = false;
   int rc = arm(b, nsteps, active);
   if (rc) return rc;
+  const int os = b->single_slot;
+  const bool stepping = os >= 0 && b->hcav[os].mode == PF_MODE_STEP;
   if (*b->hactive > 0) {
-    int64_t first = std::max<int64_t>(8, (b->predicted_rounds * 15) / 16);
+    b->obs_slot = (b->track && stepping && nsteps > 1 && *b->hactive == 1) ? os : -1;
+    // First chunk: 15/16 of the rounds the last run predicts for this one
+    // (half of them in a tracked run, whose later chunks follow the steps).
+    int64_t first = b->predicted_rounds;
+    if (stepping && b->step_rounds > 0) first = std::min<int64_t>(first, b->step_rounds * nsteps);
+    first = (first * 15) / 16;
+    if (b->obs_slot >= 0) first = std::min<int64_t>(b->max_ahead, b->step_rounds * nsteps / 2);
+    first = std::max<int64_t>(8, first);
     int64_t step = 16;
+    int64_t over = 0;  // tracked run: last doubling chunk
     rc = chunk(b, first);
     if (rc) return rc;
     while (true) {
@@ -1035,20 +1081,24 @@ int pf_batch_run(pf_batch* b, int nsteps, const int32_t* active, pf_step_diag* d
       }
       if (*b->hactive <= 0) break;
       if (b->rounds > b->round_guard) return fail(PF_ESTATE, "round limit exceeded (internal error)");
-      rc = chunk(b, step);
+      const int64_t n = (b->obs_slot >= 0) ? tracked_chunk(b, over, nsteps) : step;
+      rc = chunk(b, n);
       if (rc) return rc;
       step = std::min<int64_t>(step * 2, b->max_chunk);
     }
   }
+  b->obs_slot = -1;
   rc = pull_scal(b);
   if (rc) return rc;
   if (*b->hactive <= 0) {
     // The next run's first chunk: the rounds the slowest cavity needed this
-    // time (the launched rounds include the last chunk's overshoot).
+    // time (the launched rounds include the last chunk's overshoot), and for
+    // one stepping cavity its rounds per step.
     int64_t need = 0;
     for (int s = 0; s < b->max_cav; ++s)
       if (b->used[s]) need = std::max<int64_t>(need, b->hscal[s].rounds_run);
     if (need > 0) b->predicted_rounds = need;
+    if (stepping && b->hscal[os].steps_done > 0) b->step_rounds = b->hscal[os].rounds_run / b->hscal[os].steps_done;
   }
   if (diag)
     for (int s = 0; s < b->max_cav; ++s)
@@ -1124,6 +1174,8 @@ int pf_batch_start(pf_batch* b, int slot, int nsteps) {
   sc.it_run = 0;
   sc.outer_run = 0;
   sc.rounds_run = 0;
+  sc.step_round = 0;
+  sc.step_len = 0;
   sc.dsetup = 0;
   if (c.mode == PF_MODE_STEP) {
     sc.phase = PH_STATS;
@@ -1150,6 +1202,7 @@ int pf_batch_advance(pf_batch* b, int64_t rounds) {
   int rc = sync_tables(b);
   if (rc) return rc;
   b->serving = true;
+  b->obs_slot = -1;
   if (b->total_blocks <= 0) return PF_OK;
   return chunk(b, rounds);
 }

@@ -124,6 +124,9 @@ struct PfScal {
   int rounds_run;  // rounds this cavity executed a phase in since it was armed (deflation set-up excluded)
   int dsetup;      // the running solve is the deflation set-up, A y = 1
   int dready;      // y holds the deflation vector of this cavity's operator
+  int step_round;  // rounds_run when the last projection step ended
+  int step_len;    // rounds_run of the last projection step (the host sizes its chunks with it)
+  int pad_[2];
   double dden;     // y.A y of the stored y
   double dcoef;    // Galerkin coefficient of the finished solve
   double dt_prev, dt, cfl_now, umax, vmax, hist[5], div_before;
@@ -133,6 +136,10 @@ struct PfScal {
   double ca2, cb2;            // three-iteration round: alpha_k+2, beta_k+2
 };
 static_assert(sizeof(PfScal) % 4 == 0, "PfScal copied as 32-bit words");
+// The round kernels keep a PfScal in LDS ahead of their reduction arrays; a
+// size that is a multiple of 16 bytes keeps those arrays 16-byte aligned, so
+// their accesses compile to 128-bit LDS operations.
+static_assert(sizeof(PfScal) % 16 == 0, "reduction arrays after the LDS state stay 16-byte aligned");
 
 template <class T>
 struct Grid {
@@ -710,6 +717,8 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
       S.continuity = sqrt(tot[1]) / pymax(1.0, (double)((size_t)C.nx * C.ny));
       S.steps_done += 1;
       S.steps_left -= 1;
+      S.step_len = S.rounds_run - S.step_round;
+      S.step_round = S.rounds_run;
       if (C.mode == PF_MODE_STEP && S.steps_left > 0)
         S.phase = PH_STATS;
       else
