@@ -9,6 +9,11 @@ The PCG round dominates every workload (about 98% of the launches of a
   that each workgroup walks ``rpw`` consecutive regions;
 * ``pcg_per_round``: MIXED, 1, 2 or 3 PCG iterations per round (s-step PCG,
   s = 2 or 3), with an FP32 or (on large grids, s <= 2) an FP64 iterate.
+  Three iterations per round are a candidate only when ``PYFLOW_TUNE_S3=1``:
+  on MI355X its round costs more per iteration than the two-iteration round
+  (21.3 us for three against 13.2 us for two at 1024x1024), so it can only
+  win a timing by noise, and a 200-step 1024x1024 run that picked it was 9%
+  slower.
 
 They change where cells are computed and the order in which FP64 partial sums
 are folded, never the arithmetic of a cell. ``tune`` times each distinct
@@ -16,7 +21,9 @@ candidate on the device: a pinned-Neumann Poisson solve (the operator of
 ``solvers/pressure_solver.py:97-106`` in the reference) with a seeded N(0,1)
 right-hand side runs a few warm-up rounds, then a timed span of rounds, and the
 candidate's cost is GPU time per PCG iteration executed in that span. The
-tolerance is set so the solve cannot finish inside the span. ``tune_batch``
+tolerance is set so the solve cannot finish inside the span. The candidates
+within 10% of the fastest are timed twice more, in turn, and each keeps its
+best time, so one slow or fast span does not decide the pick. ``tune_batch``
 does the same for the region height and per-cavity workgroup cap of a whole
 batch.
 
@@ -78,12 +85,13 @@ def candidates(nx: int, ny: int, precision: str | int = "mixed",
     # PYFLOW_XD says); three iterations per round need an FP32 iterate.
     xd_env = os.environ.get("PYFLOW_XD")
     xd = (xd_env not in ("0", "")) if xd_env is not None else max(nx, ny) >= 1536
+    s3 = os.environ.get("PYFLOW_TUNE_S3") == "1"
     seen, out = set(), []
     for r in rows:
         for cap in caps:
             tpb, _, rpw, nblk = blocking(nx, ny, r, cap)
             for s in (3, 2, 1):
-                if s == 3 and not (mixed and tpb in (2, 4) and not xd):
+                if s == 3 and not (s3 and mixed and tpb in (2, 4) and not xd):
                     continue
                 if s == 2 and not (mixed and tpb <= 4):
                     continue
@@ -210,14 +218,24 @@ def tune(nx: int, ny: int, precision: str | int = "mixed", *, warmup: int = 16, 
         hit = cached(nx, ny, prec)
         if hit is not None:
             return hit
-    best = None
-    for cand in candidates(nx, ny, prec):
+    def timed(cand: Tile) -> Tile:
         t = measure(nx, ny, prec, cand, warmup=warmup, rounds=rounds)
         if log:
             log(f"[autotune] {nx}x{ny} rows={t.tile_rows} maxwg={t.max_workgroups} s={t.pcg_per_round}: "
                 f"{t.us_per_iteration:.2f} us/iteration")
-        if best is None or t.us_per_iteration < best.us_per_iteration:
-            best = t
+        return t
+
+    times = [timed(c) for c in candidates(nx, ny, prec)]
+    finite = [t.us_per_iteration for t in times if np.isfinite(t.us_per_iteration)]
+    if finite:
+        close = [k for k, t in enumerate(times) if t.us_per_iteration <= 1.1 * min(finite)]
+        if len(close) > 1:
+            for _ in range(2):
+                for k in close:
+                    t = timed(times[k])
+                    if t.us_per_iteration < times[k].us_per_iteration:
+                        times[k] = t
+    best = min(times, key=lambda t: t.us_per_iteration) if times else None
     if best is None or not np.isfinite(best.us_per_iteration):
         raise RuntimeError(f"autotune found no working candidate for {nx}x{ny}")
     key = _key(nx, ny, prec)

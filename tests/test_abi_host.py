@@ -266,21 +266,27 @@ def test_tile_fields_validated():
         assert field.encode() in L.pf_last_error()
 
 
-def test_autotune_candidates_are_distinct_and_valid():
+def test_autotune_candidates_are_distinct_and_valid(monkeypatch):
     from pyflow import autotune as A
-    for n, prec in ((1024, "mixed"), (4096, "mixed"), (96, "fp64"), (20, "mixed")):
-        cs = A.candidates(n, n, prec)
-        assert cs
-        keys = {(A.blocking(n, n, c.tile_rows, c.max_workgroups), c.pcg_per_round) for c in cs}
-        assert len(keys) == len(cs)
-        for c in cs:
-            tpb = A.blocking(n, n, c.tile_rows, c.max_workgroups)[0]
-            assert c.pcg_per_round in (1, 2, 3)
-            if c.pcg_per_round == 2:  # s-step only where the library runs it
-                assert prec == "mixed" and tpb <= 4
-            if c.pcg_per_round == 3:  # FP32 iterate (below 1536 cells a side), regions of 2 or 4 tiles
-                assert prec == "mixed" and tpb in (2, 4) and n < 1536
-        assert (prec == "mixed" and n < 1536) == any(c.pcg_per_round == 3 for c in cs), (n, prec)
+    for s3 in (False, True):
+        if s3:
+            monkeypatch.setenv("PYFLOW_TUNE_S3", "1")
+        else:
+            monkeypatch.delenv("PYFLOW_TUNE_S3", raising=False)
+        for n, prec in ((1024, "mixed"), (4096, "mixed"), (96, "fp64"), (20, "mixed")):
+            cs = A.candidates(n, n, prec)
+            assert cs
+            keys = {(A.blocking(n, n, c.tile_rows, c.max_workgroups), c.pcg_per_round) for c in cs}
+            assert len(keys) == len(cs)
+            for c in cs:
+                tpb = A.blocking(n, n, c.tile_rows, c.max_workgroups)[0]
+                assert c.pcg_per_round in (1, 2, 3)
+                if c.pcg_per_round == 2:  # s-step only where the library runs it
+                    assert prec == "mixed" and tpb <= 4
+                if c.pcg_per_round == 3:  # FP32 iterate (below 1536 cells a side), regions of 2 or 4 tiles
+                    assert prec == "mixed" and tpb in (2, 4) and n < 1536
+            # three iterations per round only on request
+            assert (s3 and prec == "mixed" and n < 1536) == any(c.pcg_per_round == 3 for c in cs), (n, prec)
 
 
 def test_serving_footprint_and_arena():
