@@ -310,11 +310,13 @@ def c1_baselines(args):
 
 
 def drop_in_c2(args, steps, tile):
-    """The 1024^2 cavity from rest for ``steps`` steps twice: through the
-    C-ABI batch run (pf_batch_run of all steps) and through the drop-in
-    SimulationDriver.run (one device step per yield). The difference is the
-    host cost the drop-in path adds per step: parameter arming, the scalar
-    state download, the diagnostics dictionaries, the centerline."""
+    """The 1024^2 cavity from rest for ``steps`` steps three times: through
+    the C-ABI batch run (pf_batch_run of all steps), through one
+    pf_batch_run per step, and through the drop-in SimulationDriver.run (one
+    device step per yield). The driver's first yield also creates the device
+    copy of the state and uploads it; later yields add parameter arming, the
+    scalar state download, the diagnostics dictionaries and the centerline,
+    measured against the per-step runs."""
     from types import SimpleNamespace
     import torch
     from pyflow import autotune
@@ -333,6 +335,13 @@ def drop_in_c2(args, steps, tile):
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     batch.close()
+    batch = DeviceBatch([params], args.precision)
+    per_run = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        batch.run(1)
+        per_run.append(time.perf_counter() - t0)
+    batch.close()
     cfg = SimpleNamespace(nx=n, ny=n, Re=1000.0, lid_velocity=1.0, cfl_target=0.5, cfl_growth=1.05,
                           advection_scheme="quick", disable_advection=False, lin_tol=1e-6, lin_maxiter=10000,
                           diagnostics=False, force_quiet=True, precision=args.precision,
@@ -340,15 +349,23 @@ def drop_in_c2(args, steps, tile):
     if tile is not None:  # the drop-in cavity runs with the batch's tiling
         cfg.tile_rows, cfg.max_workgroups, cfg.pcg_per_round = tile.tile_rows, tile.max_workgroups, tile.pcg_per_round
     state = allocate_state(n, n)
+    per_yield = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    tl = t0
     for _ in SimulationDriver(cfg, state, ResidualManager()).run(max_steps=steps):
-        pass
+        t = time.perf_counter()
+        per_yield.append(t - tl)
+        tl = t
     torch.cuda.synchronize()
     t_drv = time.perf_counter() - t0
+    later = (sum(per_yield[1:]) - sum(per_run[1:])) / max(1, steps - 1)
     return {"steps_from_rest": steps, "driver_ms_per_step": round(t_drv / steps * 1e3, 3),
             "batch_run_ms_per_step": round(t_batch / steps * 1e3, 3),
-            "host_overhead_ms_per_step": round((t_drv - t_batch) / steps * 1e3, 3),
+            "one_run_per_step_ms_per_step": round(sum(per_run) / steps * 1e3, 3),
+            "driver_first_yield_ms": round(per_yield[0] * 1e3, 3),
+            "one_run_first_step_ms": round(per_run[0] * 1e3, 3),
+            "host_overhead_ms_per_later_step": round(later * 1e3, 3),
             "driver_mcell_steps_s": round(n * n * steps / t_drv / 1e6, 3)}
 
 

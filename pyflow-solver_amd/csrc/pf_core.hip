@@ -212,8 +212,10 @@ static void choose_ca(PfCav& c, int prec, int pcg_per_round) {
     c.ca = (prec == PF_PREC_MIXED && c.tpb <= 4 && s >= 2) ? 1 : 0;
 }
 
-// Dynamic LDS of a cavity's float32 PCG round.
+// Dynamic LDS of a cavity's round: the float32 PCG region's staging, or in
+// FP64 the direction array of the in-round CG (local_cg).
 static int lds_floats_of(const PfCav& c) {
+  if (c.prec != PF_PREC_MIXED) return c.local ? 2 * c.nx * c.ny : 0;
   return c.ca == 2 ? ca3_lds_floats(c.tpb) : (c.ca ? ca_lds_floats(c.tpb) : region_lds_floats(c.tpb));
 }
 
@@ -255,6 +257,13 @@ static void fill_params(PfCav& c, const pf_cavity_params* p, int prec) {
   const int xd_dflt = xd_env >= 0 ? (xd_env ? 1 : 0) : (nmax >= 1536 ? 1 : 0);
   c.xd = (prec == PF_PREC_MIXED) ? (p->iterate64 > 0 ? (p->iterate64 == 2 ? 1 : 0) : xd_dflt) : 0;
   choose_ca(c, prec, p->pcg_per_round);
+  // FP64 cavities of one region run a solve's CG iterations inside one round
+  // (bitwise the same as one round per phase); PYFLOW_LOCAL=0 turns it off.
+  // Its direction array takes nx*ny doubles of LDS, at most 48 KiB here.
+  c.local = (prec == PF_PREC_FP64 && c.nreg == 1 && (int64_t)c.nx * c.ny <= 6144 &&
+             env_switch("PYFLOW_LOCAL", 1) != 0)
+                ? 1
+                : 0;
   const int defl_dflt = (p->mode != PF_MODE_SOLVE) ? 1 : 0;
   c.defl = (prec == PF_PREC_MIXED && p->op == PF_OP_NEG_LAP_PINNED)
                ? (p->deflate > 0 ? (p->deflate == 2 ? 1 : 0) : defl_dflt)
@@ -394,6 +403,7 @@ static RoundLaunch round_launch(const pf_batch* b) {
   } else {
     L.fn = b->unroll >= 4 ? round_fn_u<double, 4>(b) : b->unroll == 2 ? round_fn_u<double, 2>(b)
                                                                       : round_fn_u<double, 1>(b);
+    L.lds = (size_t)b->lds_floats * sizeof(float);
   }
   return L;
 }
@@ -916,6 +926,7 @@ int pf_batch_set_params(pf_batch* b, int slot, const pf_cavity_params* params) {
   nc.tpb = c.tpb;
   nc.nreg = c.nreg;
   nc.rpw = c.rpw;
+  nc.local = c.local;  // sized into the batch's LDS with that blocking
   choose_ca(nc, b->prec, params->pcg_per_round);
   b->hpar[slot] = *params;
   if (!same_operator(nc, c)) b->dreset[slot] = 1;

@@ -112,6 +112,7 @@ struct PfCav {
   double* part;                                 // [2][NPS][nblk]: slot k of workgroup b at k*nblk + b
   float* y;                                     // MIXED: deflation vector (nullptr in FP64)
   int defl;  // MIXED, pinned operator: Galerkin correction along y = A^-1 1 after each solve
+  int local;  // FP64, one region: the CG iterations of a solve run inside one round (local_cg)
 };
 
 // Scalar state of one cavity; two copies (round parity) per cavity.
@@ -541,6 +542,23 @@ __device__ __attribute__((always_inline)) inline void tail_ca3(const PfCav& C, P
   S.phase = PH_CG;
 }
 
+// Tails of the FP64 CG phases P and Q, shared by the generic tail and the
+// in-round CG (local_cg); Q's phase transition is cg_next's.
+__device__ __attribute__((always_inline)) inline void tail_p(PfScal& S, const double* tot) {
+  S.pq = tot[0];
+  S.alpha = S.rho / tot[0];
+  S.phase = PH_Q;
+}
+__device__ __attribute__((always_inline)) inline void tail_q_sums(PfScal& S, const double* tot) {
+  S.rho_prev = S.rho;
+  S.rho = tot[0];
+  S.rnorm2 = tot[1];
+  S.beta = S.rho / S.rho_prev;
+  S.it += 1;
+  S.it_total += 1;
+  S.it_run += 1;
+}
+
 // V tells instances apart: each kernel instance that calls the cold functions
 // gets its own copies, compiled within that kernel's register budget.
 template <int V>
@@ -625,19 +643,9 @@ __attribute__((noinline)) __device__ void tail(const PfCav& C, PfScal& S, const 
       S.phase = (sqrt(S.rnorm2) < S.atol) ? PH_FIN : PH_P;
       break;
     }
-    case PH_P:
-      S.pq = tot[0];
-      S.alpha = S.rho / tot[0];
-      S.phase = PH_Q;
-      break;
+    case PH_P: tail_p(S, tot); break;
     case PH_Q:
-      S.rho_prev = S.rho;
-      S.rho = tot[0];
-      S.rnorm2 = tot[1];
-      S.beta = S.rho / S.rho_prev;
-      S.it += 1;
-      S.it_total += 1;
-      S.it_run += 1;
+      tail_q_sums(S, tot);
       cg_next(C, S);
       break;
     case PH_OUTER_A: S.phase = PH_OUTER_B; break;
@@ -2197,6 +2205,164 @@ __attribute__((noinline)) __device__ void round_cold(int phase, const PfCav* __r
   if (phase_nparts(phase) > 0) block_partials_phase(phase, acc, sm, part_out, pstride);
 }
 
+// block_partials_k<K, 0> with the workgroup totals left in out[k] (LDS)
+// instead of the partials array: the same reduction tree, the same bits.
+template <int K>
+__device__ inline void block_total(double (&v)[NP], double (*sm)[NPS], double* out) {
+  const int tid = tid_flat();
+  const int lane = tid & 63, wave = tid >> 6;
+  wave_reduce_k<K, 0u>(v);
+  if (lane == 63) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[wave][k] = v[k];
+  }
+  __syncthreads();
+  if (tid < K) {
+    double t = sm[0][tid];
+#pragma unroll
+    for (int w = 1; w < NWAVE; ++w) t = t + sm[w][tid];
+    out[tid] = t;
+  }
+  __syncthreads();
+}
+
+// FP64 cavities of one region (PfCav::local; one workgroup holds every cell):
+// the CG iterations of a solve run inside the round that starts the solve's
+// P phase, instead of one round per P or Q phase. A thread keeps x, r and
+// q = A d of its cells in registers (q is formed once, in P; the Q round forms
+// it again from the same values, the same bits); the direction d lives in
+// LDS, where the stencil reads its neighbours. Each phase is computed as the round of that phase
+// computes it: the same cell expressions (dnew_at, apply_op5, chunk_q), the
+// same per-thread order of the sums, the same block reduction (block_total),
+// the totals folded as the next round folds one workgroup's partials
+// (0.0 + t), and the same tails (tail_p, tail_q_sums, cg_next's choice). The
+// results are therefore the bits of the round-per-phase path. The loop ends
+// after a Q phase whose tail would leave the iteration: that Q's partials
+// are stored and the state is left pending on Q, as after a Q round, so the
+// next round runs the tail and what follows it. Bounded by lin_maxiter. A
+// solve whose residual is NaN runs one round per phase instead.
+// MC: cells per thread, at least the region's tiles (4 or 8).
+template <class T, int MC>
+__device__ __attribute__((noinline)) void local_cg(const PfCav* __restrict__ Cg, PfScal& st, double (*sm)[NPS],
+                                                   double* tot, double* __restrict__ lD, double* part_out) {
+  static_assert(sizeof(T) == sizeof(double), "FP64 cavities only");
+  const PfCav& C = *Cg;
+  __shared__ double raw[2];
+  __shared__ int go;
+  // The descriptor's fields, read once (a load from global memory after every
+  // tail would sit on the iteration's critical path).
+  const int nx = C.nx, ny = C.ny;
+  const int nt = C.tiles_y, op = C.op, jac = C.jacobi, maxiter = C.maxiter, nblk = C.nblk;
+  const double invd = C.invd;
+  const T dx = (T)C.dx, dy = (T)C.dy;
+  const int i = threadIdx.x, lj = threadIdx.y;
+  const int tid = tid_flat();
+  T* X = (T*)C.x;
+  T* R = (T*)C.r;
+  T* const D0 = (T*)C.d0;
+  T* const D1 = (T*)C.d1;
+  T xv[MC], rv[MC], qv[MC];
+  {
+    const T* Dold = (st.it & 1) ? D0 : D1;
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+      const int j = t * TY + lj;
+      const bool ok = (t < nt) && (i < nx) && (j < ny);
+      const int k = ok ? j * nx + i : 0;
+      xv[t] = ok ? X[k] : T(0);
+      rv[t] = ok ? R[k] : T(0);
+      qv[t] = T(0);
+      if (ok) lD[k] = Dold[k];  // the previous direction (read back by this thread only)
+    }
+  }
+  auto cell_ok = [&](int t, int& j, int& k) {
+    j = t * TY + lj;
+    k = j * nx + i;
+    return (t < nt) && (i < nx) && (j < ny);
+  };
+  auto q_at = [&](int j, int k) -> T {
+    const T c = lD[k];
+    const T e = (i + 1 < nx) ? lD[k + 1] : T(0);
+    const T w = (i > 0) ? lD[k - 1] : T(0);
+    const T n = (j + 1 < ny) ? lD[k + nx] : T(0);
+    const T s = (j > 0) ? lD[k - nx] : T(0);
+    return apply_op5<T>(op, c, e, w, n, s, nx, ny, j, i, dx, dy);
+  };
+  while (true) {
+    // P (chunk_p): d = M r (first iteration) or d_old * beta + M r; q = A d.
+    const int first = (st.it == 0);
+    const double beta = st.beta;
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+      int j, k;
+      if (cell_ok(t, j, k)) {
+        const T z = (T)jacobi_minv(jac, invd, nx, ny, j, i) * rv[t];
+        lD[k] = first ? z : (T)((double)lD[k] * beta + (double)z);
+      }
+    }
+    __syncthreads();
+    double acc[NP] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+      int j, k;
+      if (cell_ok(t, j, k)) {
+        qv[t] = q_at(j, k);
+        acc[0] += (double)lD[k] * (double)qv[t];
+      }
+    }
+    block_total<1>(acc, sm, raw);  // its barriers also end this phase's LDS reads
+    if (tid == 63) {
+      tot[0] = 0.0 + raw[0];
+      tail_p(st, tot);  // alpha = rho / (d.q); phase Q
+    }
+    __syncthreads();
+    // Q (chunk_q): x += alpha d, r -= alpha q, partials r.(M r) and r.r.
+    const double alpha = st.alpha;
+    double acq[NP] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+      int j, k;
+      if (cell_ok(t, j, k)) {
+        xv[t] = (T)((double)xv[t] + alpha * (double)lD[k]);
+        rv[t] = (T)((double)rv[t] - alpha * (double)qv[t]);
+        const T zw = (T)jacobi_minv(jac, invd, nx, ny, j, i);
+        const T z = zw * rv[t];
+        acq[0] += (double)rv[t] * (double)z;
+        acq[1] += (double)rv[t] * (double)rv[t];
+      }
+    }
+    block_total<2>(acq, sm, raw);
+    if (tid == 63) {
+      tot[0] = 0.0 + raw[0];
+      tot[1] = 0.0 + raw[1];
+      // cg_next for FP64 after the Q tail: another P unless the budget is
+      // spent or the residual meets the inner tolerance. A NaN residual also
+      // leaves the loop: such a solve goes on one round per phase (the same
+      // bits), so a host that watches for NaN sees it between rounds.
+      const bool more = (st.it_total + 1 < maxiter) && !(sqrt(tot[1]) < st.inner_atol) && !isnan(tot[1]);
+      if (more) {
+        tail_q_sums(st, tot);
+        st.phase = PH_P;  // cg_next's choice when more holds
+      }
+      go = more ? 1 : 0;
+    }
+    __syncthreads();
+    if (!go) break;
+  }
+  // Leave the vectors and partials as a Q round leaves them.
+  T* Dn = (st.it & 1) ? D1 : D0;  // the last P's direction array
+#pragma unroll
+  for (int t = 0; t < MC; ++t) {
+    int j, k;
+    if (cell_ok(t, j, k)) {
+      X[k] = xv[t];
+      R[k] = rv[t];
+      Dn[k] = lD[k];
+    }
+  }
+  if (tid < 2) part_out[(size_t)tid * nblk] = raw[tid];
+}
+
 // Body of one round for workgroup b of cavity c. M > 0 (float32 only): the
 // cavity's region height in tiles. (Issuing the PCG region loads at entry,
 // before the phase is known, gained no time and gave wrong results in
@@ -2319,6 +2485,23 @@ __device__ __attribute__((always_inline)) inline void round_body(const KArgs& KA
       }
     }
     block_partials_k<(XD ? KXD : 3), 0u>(acc, sm, part_out, nblk);
+  } else if (M == 0 && sizeof(T) == sizeof(double) && phase == PH_P && C.local && !isnan(st.rnorm2)) {
+    if constexpr (sizeof(T) == sizeof(double)) {
+      if (C.tiles_y <= 4)
+        local_cg<T, 4>(Cg, st, sm, tot, (double*)lds, part_out);
+      else
+        local_cg<T, 8>(Cg, st, sm, tot, (double*)lds, part_out);
+    }
+    if (b == 0) {
+      __syncthreads();
+      if (tid == 0) {
+        st.pending = PH_Q;
+        if (!st.dsetup) st.rounds_run += 1;
+      }
+      __syncthreads();
+      if (tid < NW32) dst[tid] = ((const uint32_t*)&st)[tid];
+    }
+    return;
   } else if (M == 0 && (phase == PH_P || phase == PH_Q)) {
     for (int r = r0; r < r1; ++r) {
       const int ty0 = (r / C.tiles_x) * C.tpb;
