@@ -744,10 +744,12 @@ int pf_batch_create_empty(int precision, int max_cav, int64_t max_tiles, void* s
   b->hpar.assign(max_cav, pf_cavity_params{});
   b->unroll = env_int("PYFLOW_UNROLL", 2);
   b->graph_len = graph_len_of(env_switch("PYFLOW_GRAPH", 64));
-  // Chunks double up to max_chunk rounds between host checks: each check idles
-  // the GPU while the host queues the next chunk, and the last chunk of a run
-  // overshoots by at most its length (empty rounds). 1024 against 256: 0.6%
-  // faster over the 200 steps of the 1024^2 cavity.
+  // Untracked runs (batches, polls) double their chunks up to max_chunk rounds
+  // between host checks: each check idles the GPU while the host queues the
+  // next chunk, and the last chunk of a run overshoots by at most its length
+  // (empty rounds). 1024 against 256 made the 200 steps of the 1024^2 cavity
+  // 0.6% faster before that run was tracked. A tracked run (one cavity, several
+  // steps) queues at most max_ahead rounds on a prediction (tracked_chunk).
   b->max_chunk = std::max<int64_t>(16, env_int("PYFLOW_MAX_CHUNK", 1024));
   b->track = env_switch("PYFLOW_TRACK", 1) != 0;
   b->max_ahead = std::max<int64_t>(16, env_int("PYFLOW_MAX_AHEAD", 4096));
