@@ -2229,19 +2229,20 @@ __device__ inline void block_total(double (&v)[NP], double (*sm)[NPS], double* o
 // FP64 cavities of one region (PfCav::local; one workgroup holds every cell):
 // the CG iterations of a solve run inside the round that starts the solve's
 // P phase, instead of one round per P or Q phase. A thread keeps x, r and
-// q = A d of its cells in registers (q is formed once, in P; the Q round forms
-// it again from the same values, the same bits); the direction d lives in
-// LDS, where the stencil reads its neighbours. Each phase is computed as the round of that phase
-// computes it: the same cell expressions (dnew_at, apply_op5, chunk_q), the
-// same per-thread order of the sums, the same block reduction (block_total),
-// the totals folded as the next round folds one workgroup's partials
-// (0.0 + t), and the same tails (tail_p, tail_q_sums, cg_next's choice). The
-// results are therefore the bits of the round-per-phase path. The loop ends
-// after a Q phase whose tail would leave the iteration: that Q's partials
-// are stored and the state is left pending on Q, as after a Q round, so the
-// next round runs the tail and what follows it. Bounded by lin_maxiter. A
-// solve whose residual is NaN runs one round per phase instead.
-// MC: cells per thread, at least the region's tiles (4 or 8).
+// q = A d of its cells in registers (q is formed once, in P; the Q round
+// forms it again from the same values, the same bits); the direction d lives
+// in LDS, where the stencil reads its neighbours. Each phase is computed as
+// the round of that phase computes it: the same cell expressions (dnew_at,
+// apply_op5, chunk_q), the same per-thread order of the sums, the same block
+// reduction (block_total), the totals folded as the next round folds one
+// workgroup's partials (0.0 + t), and the same tails (tail_p, tail_q_sums,
+// cg_next's choice). The results are therefore the bits of the
+// round-per-phase path. The loop ends after a Q phase whose tail would leave
+// the iteration: that Q's partials are stored and the state is left pending
+// on Q, as after a Q round, so the next round runs the tail and what follows
+// it. Bounded by lin_maxiter. A solve whose residual is NaN runs one round
+// per phase instead. MC: cells per thread, at least the region's tiles (4 or
+// 8).
 template <class T, int MC>
 __device__ __attribute__((noinline)) void local_cg(const PfCav* __restrict__ Cg, PfScal& st, double (*sm)[NPS],
                                                    double* tot, double* __restrict__ lD, double* part_out) {
